@@ -1,0 +1,162 @@
+"""Benchmark: Gsites/s of the 5-loop F_munu + B^2 (magnetic) measurement on a 48^3x96
+lattice, FP32 links with FP64 sums versus the plain FP64 path, on one MI355X.
+
+A step is one full measurement of B over the lattice (the field-strength kernel for
+the three spatial planes fused with the -ReTr F^2 reduction, plus the fixed-order
+second pass), with the links already resident in HBM in the SoA layout.  The field is
+synthetic: the seeded SU(3) generator with eps = 0.3, generated in FP64 on the GPU
+and rounded to FP32 for the FP32 path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  Only one GPU is used: --gpus N > 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DIMS = (96, 48, 48, 48)      # (NT, NX, NY, NZ)
+SEED = 1003
+EPS = 0.3
+# Algorithmic work per site of the magnetic measurement: 258 SU(3) products of 198 flop
+# (28 leaves x 3 products x 3 planes + 2 line products x 3 directions) and the
+# compulsory read of the three spatial links.
+FLOP_PER_SITE = 258 * 198
+BYTES_PER_SITE = {"fp32": 3 * 72, "fp64": 3 * 144}
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (FP32 counts packed FMA)
+PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "magnetic_fp32_48x96_traffic.json")
+
+
+def time_steps(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return ev0.elapsed_time(ev1) / 1e3 / steps, wall / steps
+
+
+def cpu_baseline(nslices=4):
+    """The CPU oracle (FP64, OpenMP) on a bounded sample of the same workload: nslices
+    time slices of a 48^3 x nslices lattice from the same generator (B uses spatial
+    planes only, so each slice is the same work as a slice of the full lattice)."""
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    dims = (nslices,) + DIMS[1:]
+    U = O.random_field(dims, SEED, EPS)
+    t0 = time.perf_counter()
+    O.magnetic_sum_tslices(U, 1, nslices)
+    dt = time.perf_counter() - t0
+    sites = nslices * DIMS[1] * DIMS[2] * DIMS[3]
+    return {"value": sites / dt / 1e9, "unit": "Gsites/s", "cores": threads, "kind": "port",
+            "sample": f"oracle magnetic (FP64, {threads} OpenMP threads) on {nslices} time slices of 48^3: "
+                      f"{sites} sites in {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the path runs on a single MI355X (see DESIGN.md for how it shards); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import flue_lat_amd as FL
+
+    torch.cuda.set_device(0)
+    V = DIMS[0] * DIMS[1] * DIMS[2] * DIMS[3]
+    f64 = FL.GaugeField.random(DIMS, SEED, EPS, "fp64")
+    f32 = f64.astype("fp32")
+    torch.cuda.synchronize()
+
+    res = {}
+    for name, g in (("fp32", f32), ("fp64", f64)):
+        out = torch.empty(1, dtype=torch.float64, device="cuda")
+        g.workspace()
+        holder = {}
+
+        def step(g=g, holder=holder):
+            holder["B"] = g.magnetic(sync=False)
+
+        t_ev, t_wall = time_steps(step, a.steps, a.warmup, torch)
+        B = holder["B"].item()
+        res[name] = {"s_per_step": t_ev, "wall_s_per_step": t_wall, "B": B, "gsites": V / t_ev / 1e9}
+        del out
+
+    # companion observables of the same configuration (plaquette, Polyakov), FP32 vs FP64
+    extras = {}
+    for name, g in (("fp32", f32), ("fp64", f64)):
+        s, n = g.plaquette(1, 4, 4)
+        p, npl = g.polyakov()
+        extras[name] = {"plaquette": s / (3 * n), "polyakov": p / (3 * npl)}
+    rel_B = abs(res["fp32"]["B"] - res["fp64"]["B"]) / abs(res["fp64"]["B"])
+
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    t32 = res["fp32"]["s_per_step"]
+    achieved_tflops = FLOP_PER_SITE * V / t32 / 1e12
+    achieved_gbs = BYTES_PER_SITE["fp32"] * V / t32 / 1e9
+    line = {
+        "metric": "Gsites/s for 5-loop Fmunu+B^2 at 48^3x96, FP32-link vs FP64, % of HBM roofline",
+        "value": round(res["fp32"]["gsites"], 6),
+        "unit": "Gsites/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t32 * 1e3, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 links, f64 sums",
+        "data": "synthetic (seeded SU(3) field, eps=0.3, generated on the GPU)",
+        "config": {"workload": "magnetic B (5-loop F_munu on the 3 spatial planes + sum of -ReTr F^2) on 48^3x96",
+                   "lattice": "48^3x96", "sites": V, "eps": EPS, "seed": SEED},
+        "fp64": {"value": round(res["fp64"]["gsites"], 6), "ms_per_step": round(res["fp64"]["s_per_step"] * 1e3, 6)},
+        "fp32_over_fp64": round(res["fp32"]["gsites"] / res["fp64"]["gsites"], 4),
+        "rel_err_B_fp32_vs_fp64": rel_B,
+        "B": {"fp32": res["fp32"]["B"], "fp64": res["fp64"]["B"]},
+        "observables": extras,
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved_tflops, 3),
+            "peak": PEAK_TFLOPS["fp32"],
+            "unit": "TFLOP/s",
+            "frac": round(achieved_tflops / PEAK_TFLOPS["fp32"], 4),
+            "traffic": traffic,
+            "flop_per_site": FLOP_PER_SITE,
+            "hbm": {"achieved": round(achieved_gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(achieved_gbs / PEAK_HBM_GBS, 5), "bytes_per_site": BYTES_PER_SITE["fp32"]},
+            "fp64": {"achieved": round(FLOP_PER_SITE * V / res["fp64"]["s_per_step"] / 1e12, 3),
+                     "peak": PEAK_TFLOPS["fp64"],
+                     "frac": round(FLOP_PER_SITE * V / res["fp64"]["s_per_step"] / 1e12 / PEAK_TFLOPS["fp64"], 4)},
+        },
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,23 @@
+"""flue_lat_amd -- MI355X implementation of FLUE's gluonic-observable path.
+
+The module tree mirrors the reference's Python package (python/FLUE/__init__.py:1-4):
+``gluonCalc`` (with the observables in ``gluonCalc.wloops``), ``gluonAna`` and
+``compiled.flue_c``.  A drop-in user writes ``import flue_lat_amd as FLUE``.
+The compute path is libflue_hip.so (HIP kernels for gfx950, C ABI in
+include/flue_c.h); nothing here falls back to the CPU.
+"""
+from . import compiled, gluonAna, gluonCalc
+from ._lib import FlueError, FlueNotBuilt
+
+__version__ = "0.1.0"  # src/FLUE.f90:15
+
+
+def __getattr__(name):
+    # the device classes import torch; load them on first use
+    if name == "GaugeField":
+        from .field import GaugeField
+        return GaugeField
+    raise AttributeError(name)
+
+
+__all__ = ["gluonAna", "gluonCalc", "compiled", "GaugeField", "FlueError", "FlueNotBuilt"]

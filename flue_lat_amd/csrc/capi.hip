@@ -1,0 +1,532 @@
+// capi.hip -- the extern "C" surface of libflue_hip.so (declared in include/flue_c.h).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "../../include/flue_c.h"
+#include "flue_launch.hpp"
+
+using namespace flue;
+
+namespace {
+
+constexpr uint64_t kMaxSites = (1ull << 31) - 1;
+
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+Workspace g_ws[64];
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return FLUE_OK;
+    if (e == hipErrorOutOfMemory) return FLUE_ENOMEM;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return FLUE_ENODEV;
+    return FLUE_EHIP;
+}
+
+#define FLUE_TRY(expr)                  \
+    do {                                \
+        hipError_t e__ = (expr);        \
+        if (e__ != hipSuccess) return hip_status(e__); \
+    } while (0)
+
+int check_dims(const int32_t* d) {
+    if (!d) return FLUE_ENULL;
+    uint64_t v = 1;
+    for (int i = 0; i < 4; ++i) {
+        if (d[i] < 1) return FLUE_EDIM;
+        v *= (uint64_t)d[i];
+        if (v > kMaxSites) return FLUE_EDIM;
+    }
+    return FLUE_OK;
+}
+
+int check_prec(int32_t p) { return (p == FLUE_F64 || p == FLUE_F32) ? FLUE_OK : FLUE_EINVAL; }
+
+int check_path(const int32_t* path, int32_t npath) {
+    if (npath < 0 || npath > 64) return FLUE_EPATH;
+    if (npath > 0 && !path) return FLUE_ENULL;
+    for (int i = 0; i < npath; ++i)
+        if (path[i] == 0 || path[i] > 4 || path[i] < -4) return FLUE_EPATH;
+    return FLUE_OK;
+}
+
+// workspace: caller-provided, or the library's per-device buffer
+int get_ws(void* ws, size_t ws_bytes, size_t need, void** out) {
+    if (ws) {
+        if (ws_bytes < need) return FLUE_EWS;
+        *out = ws;
+        return FLUE_OK;
+    }
+    int dev = 0;
+    FLUE_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return FLUE_EINVAL;
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    Workspace& w = g_ws[dev];
+    if (w.bytes < need) {
+        if (w.ptr) FLUE_TRY(hipFree(w.ptr));
+        w.ptr = nullptr;
+        w.bytes = 0;
+        FLUE_TRY(hipMalloc(&w.ptr, need));
+        w.bytes = need;
+    }
+    *out = w.ptr;
+    return FLUE_OK;
+}
+
+size_t partial_bytes(uint32_t nblocks) { return (size_t)nblocks * 2 * sizeof(double) + 256; }
+
+hipStream_t S(void* s) { return (hipStream_t)s; }
+
+int fortran_strides(const int32_t d[4], int64_t s[7]) {
+    const int64_t V = (int64_t)d[0] * d[1] * d[2] * d[3];
+    s[0] = 1;
+    s[1] = d[0];
+    s[2] = (int64_t)d[0] * d[1];
+    s[3] = (int64_t)d[0] * d[1] * d[2];
+    s[4] = V;
+    s[5] = 4 * V;
+    s[6] = 12 * V;
+    return FLUE_OK;
+}
+
+// RAII device buffer for the host drop-ins
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// host drop-in prologue: dims by reference, field copied to the device as FP64 SoA
+struct HostField {
+    int32_t dims[4];
+    Lat L;
+    DevBuf raw, soa;
+};
+
+int load_host_field(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                    const flue_c128* data, HostField& h) {
+    if (!NT || !NX || !NY || !NZ || !data) return FLUE_ENULL;
+    h.dims[0] = *NT;
+    h.dims[1] = *NX;
+    h.dims[2] = *NY;
+    h.dims[3] = *NZ;
+    int rc = check_dims(h.dims);
+    if (rc) return rc;
+    h.L = make_lat(h.dims);
+    const size_t n = (size_t)h.L.V * 36;
+    FLUE_TRY(hipMalloc(&h.raw.p, n * sizeof(flue_c128)));
+    FLUE_TRY(hipMalloc(&h.soa.p, flue_field_bytes(h.dims, FLUE_F64)));
+    FLUE_TRY(hipMemcpy(h.raw.p, data, n * sizeof(flue_c128), hipMemcpyHostToDevice));
+    int64_t st[7];
+    fortran_strides(h.dims, st);
+    FLUE_TRY(launch_import(h.L, h.raw.p, 0, st, h.soa.p, 0, nullptr));
+    return FLUE_OK;
+}
+
+int fetch_scalar(const double* d, double* h) {
+    FLUE_TRY(hipMemcpy(h, d, sizeof(double), hipMemcpyDeviceToHost));
+    return FLUE_OK;
+}
+
+int plaquette_count(const int32_t dims[4], int32_t mu0, int32_t mu1, int32_t nu1, int32_t* nP) {
+    int64_t planes = 0;
+    for (int mu = mu0; mu <= mu1; ++mu)
+        for (int nu = mu + 1; nu <= nu1; ++nu) ++planes;
+    const int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
+    const int64_t n = planes * V;
+    if (n > INT32_MAX) return FLUE_EDIM;  // nP is a default integer in the reference
+    *nP = (int32_t)n;
+    return FLUE_OK;
+}
+
+int check_plaq_args(int32_t muStart, int32_t muEnd, int32_t nuEnd) {
+    if (muStart < 1 || muStart > 4 || muEnd < 1 || muEnd > 4 || nuEnd < 1 || nuEnd > 4) return FLUE_EINVAL;
+    return FLUE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* flue_strerror(int s) {
+    switch (s) {
+    case FLUE_OK: return "ok";
+    case FLUE_EINVAL: return "argument value out of range";
+    case FLUE_ENULL: return "required pointer is NULL";
+    case FLUE_EPATH: return "path entry must be +-1..+-4 and the path at most 64 steps";
+    case FLUE_ENT: return "Polyakov loop needs NT >= 3";
+    case FLUE_EDIM: return "lattice extents must be >= 1 and the volume below 2^31 sites";
+    case FLUE_EHIP: return "HIP runtime error";
+    case FLUE_ENOMEM: return "device allocation failed";
+    case FLUE_ENOTIMPL: return "outside the scope of this library";
+    case FLUE_ENODEV: return "no GPU visible";
+    case FLUE_EWS: return "workspace too small";
+    default: return "unknown status";
+    }
+}
+
+int flue_version(int32_t* major, int32_t* minor, int32_t* patch) {
+    if (!major || !minor || !patch) return FLUE_ENULL;
+    *major = 0;
+    *minor = 1;
+    *patch = 0;
+    return FLUE_OK;
+}
+
+// ---------------------------------------------------------------- host helpers
+
+int multiplymatmat_c(flue_c128 MM[9], const flue_c128 left[9], const flue_c128 right[9]) {
+    if (!MM || !left || !right) return FLUE_ENULL;
+    flue_c128 r[9];
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            double re = 0, im = 0;
+            for (int k = 0; k < 3; ++k) {
+                const flue_c128 a = left[i + 3 * k], b = right[k + 3 * j];
+                re += a.re * b.re - a.im * b.im;
+                im += a.re * b.im + a.im * b.re;
+            }
+            r[i + 3 * j] = {re, im};
+        }
+    std::memcpy(MM, r, sizeof r);
+    return FLUE_OK;
+}
+
+int get_qhat_c(const double coord[4], const int32_t shape[4], double qhat[4], const double* a) {
+    if (!coord || !shape || !qhat) return FLUE_ENULL;
+    const double lat = a ? *a : 1.0;
+    for (int i = 0; i < 4; ++i) {
+        if (shape[i] == 0) return FLUE_EINVAL;
+        qhat[i] = (2.0 * M_PI / lat) * coord[i] / (double)shape[i];
+    }
+    return FLUE_OK;
+}
+
+int scalargluonprop_c(const int32_t*, const int32_t*, const flue_c128*, const double*, double*) {
+    return FLUE_ENOTIMPL;
+}
+int calc_mom_space_scalard_c(const int32_t*, const int32_t*, const flue_c128*, flue_c128*, const int32_t*,
+                             const double*) {
+    return FLUE_ENOTIMPL;
+}
+int q_average_c(const int32_t*, const double*, const double*, const int32_t*, double*, double*, int32_t*) {
+    return FLUE_ENOTIMPL;
+}
+int cone_cut_c(const int32_t*, const int32_t*, const double*, const double*, const double*, const int32_t*,
+               const int32_t*, const double*, const double*, const double*, const double*, double*, double*,
+               double*, int32_t*) {
+    return FLUE_ENOTIMPL;
+}
+
+int periodcoord_c(const int32_t coord[4], const int32_t dataShape[7], int32_t out[4]) {
+    if (!coord || !dataShape || !out) return FLUE_ENULL;
+    for (int c = 0; c < 4; ++c) {
+        out[c] = coord[c];
+        if (coord[c] > dataShape[c])
+            out[c] = 1;
+        else if (coord[c] == 0)
+            out[c] = dataShape[c];
+    }
+    return FLUE_OK;
+}
+
+// ---------------------------------------------------------------- device entry points
+
+size_t flue_field_bytes(const int32_t dims[4], int32_t prec) {
+    if (check_dims(dims) || check_prec(prec)) return 0;
+    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
+    return (size_t)V * kPlanes * (prec == FLUE_F32 ? 4 : 8);
+}
+
+size_t flue_workspace_bytes(const int32_t dims[4]) {
+    if (check_dims(dims)) return 0;
+    const Lat L = make_lat(dims);
+    uint32_t nb = plaquette_blocks(L);
+    nb = std::max(nb, polyakov_blocks(L));
+    nb = std::max(nb, path_blocks(L, -1));
+    nb = std::max(nb, magnetic_blocks(L, 0));
+    nb = std::max(nb, magnetic_blocks(L, 1));
+    return partial_bytes(nb);
+}
+
+int flue_import_field(const int32_t dims[4], const void* src, int32_t src_type, const int64_t strides[7], void* dst,
+                      int32_t prec, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!src || !dst || !strides) return FLUE_ENULL;
+    if (check_prec(prec) || (src_type != FLUE_C128 && src_type != FLUE_C64)) return FLUE_EINVAL;
+    return hip_status(launch_import(make_lat(dims), src, src_type == FLUE_C64, strides, dst, prec == FLUE_F32, S(stream)));
+}
+
+int flue_export_field(const int32_t dims[4], const void* src, int32_t prec, void* dst, int32_t dst_type,
+                      const int64_t strides[7], void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!src || !dst || !strides) return FLUE_ENULL;
+    if (check_prec(prec) || (dst_type != FLUE_C128 && dst_type != FLUE_C64)) return FLUE_EINVAL;
+    return hip_status(launch_export(make_lat(dims), src, prec == FLUE_F32, dst, dst_type == FLUE_C64, strides, S(stream)));
+}
+
+int flue_convert_field(const int32_t dims[4], const void* src, int32_t src_prec, void* dst, int32_t dst_prec,
+                       void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!src || !dst) return FLUE_ENULL;
+    if (check_prec(src_prec) || check_prec(dst_prec)) return FLUE_EINVAL;
+    return hip_status(launch_convert(make_lat(dims), src, src_prec == FLUE_F32, dst, dst_prec == FLUE_F32, S(stream)));
+}
+
+int flue_random_field(const int32_t dims[4], uint64_t seed, double eps, void* dst, int32_t prec, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!dst) return FLUE_ENULL;
+    if (check_prec(prec) || !std::isfinite(eps)) return FLUE_EINVAL;
+    return hip_status(launch_random_field(make_lat(dims), seed, eps, dst, prec == FLUE_F32, S(stream)));
+}
+
+int flue_plaquette(const int32_t dims[4], const void* U, int32_t prec, int32_t muStart, int32_t muEnd, int32_t nuEnd,
+                   double* d_sum, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !d_sum) return FLUE_ENULL;
+    if (check_prec(prec) || check_plaq_args(muStart, muEnd, nuEnd)) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    const uint32_t nb = plaquette_blocks(L);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
+    FLUE_TRY(launch_plaquette(L, U, prec == FLUE_F32, muStart - 1, muEnd - 1, nuEnd - 1, (double*)w, S(stream)));
+    FLUE_TRY(launch_finalize((double*)w, nb, d_sum, 1.0, S(stream)));
+    return FLUE_OK;
+}
+
+int flue_polyakov(const int32_t dims[4], const void* U, int32_t prec, double* d_sum, void* ws, size_t ws_bytes,
+                  void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !d_sum) return FLUE_ENULL;
+    if (check_prec(prec)) return FLUE_EINVAL;
+    if (dims[0] < 3) return FLUE_ENT;
+    const Lat L = make_lat(dims);
+    const uint32_t nb = polyakov_blocks(L);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
+    FLUE_TRY(launch_polyakov(L, U, prec == FLUE_F32, (double*)w, S(stream)));
+    FLUE_TRY(launch_finalize((double*)w, nb, d_sum, 1.0, S(stream)));
+    return FLUE_OK;
+}
+
+int flue_path_trace(const int32_t dims[4], const void* U, int32_t prec, const int32_t* path, int32_t npath,
+                    int32_t tau, double* d_sum, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !d_sum) return FLUE_ENULL;
+    if (check_prec(prec)) return FLUE_EINVAL;
+    if ((rc = check_path(path, npath))) return rc;
+    if (npath < 1) return FLUE_EPATH;
+    if (tau < 0 || tau > dims[0]) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    const int ts = tau - 1;  // -1 = all slices
+    const uint32_t nb = path_blocks(L, ts);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
+    FLUE_TRY(launch_path_trace(L, U, prec == FLUE_F32, path, npath, ts, (double*)w, S(stream)));
+    FLUE_TRY(launch_finalize((double*)w, nb, d_sum, 1.0, S(stream)));
+    return FLUE_OK;
+}
+
+int flue_generic_path(const int32_t dims[4], const void* U, int32_t prec, const int32_t coordBase[4],
+                      const int32_t* path, int32_t npath, double* d_out, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !d_out || !coordBase) return FLUE_ENULL;
+    if (check_prec(prec)) return FLUE_EINVAL;
+    if ((rc = check_path(path, npath))) return rc;
+    int32_t base0[4];
+    for (int i = 0; i < 4; ++i) {
+        if (coordBase[i] < 1 || coordBase[i] > dims[i]) return FLUE_EINVAL;
+        base0[i] = coordBase[i] - 1;
+    }
+    return hip_status(launch_generic_path(make_lat(dims), U, prec == FLUE_F32, base0, path, npath, d_out, S(stream)));
+}
+
+int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, int32_t nu, int32_t variant, void* out,
+               int32_t layout, const int64_t strides[6], void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !out) return FLUE_ENULL;
+    if (check_prec(prec) || mu < 1 || mu > 4 || nu < 1 || nu > 4 || (variant != 0 && variant != 1))
+        return FLUE_EINVAL;
+    if (layout != FLUE_OUT_STRIDED && layout != FLUE_OUT_SOA) return FLUE_EINVAL;
+    if (layout == FLUE_OUT_STRIDED && !strides) return FLUE_ENULL;
+    return hip_status(launch_fmunu(make_lat(dims), U, prec == FLUE_F32, mu - 1, nu - 1, variant, out,
+                                   layout == FLUE_OUT_SOA, strides, S(stream)));
+}
+
+int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_B, void* ws, size_t ws_bytes,
+                  void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !d_B) return FLUE_ENULL;
+    if (check_prec(prec)) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    const uint32_t nb = magnetic_blocks(L, prec == FLUE_F32);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
+    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, (double*)w, S(stream)));
+    FLUE_TRY(launch_finalize((double*)w, nb, d_B, 1.0 / (double)L.V, S(stream)));
+    return FLUE_OK;
+}
+
+// ---------------------------------------------------------------- host drop-ins
+
+int genericpath_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+                  const int32_t coordBase[4], const int32_t* npath, const int32_t* path, flue_c128 out[9]) {
+    if (!npath || !coordBase || !out) return FLUE_ENULL;
+    int rc = check_path(path, *npath);
+    if (rc) return rc;
+    HostField h;
+    if ((rc = load_host_field(NT, NX, NY, NZ, data, h))) return rc;
+    DevBuf d;
+    FLUE_TRY(hipMalloc(&d.p, 18 * sizeof(double)));
+    if ((rc = flue_generic_path(h.dims, h.soa.p, FLUE_F64, coordBase, path, *npath, (double*)d.p, nullptr)))
+        return rc;
+    FLUE_TRY(hipMemcpy(out, d.p, 18 * sizeof(double), hipMemcpyDeviceToHost));
+    return FLUE_OK;
+}
+
+static int plaq_host(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                     const flue_c128* data, const int32_t* muStart, const int32_t* muEnd, const int32_t* nuEnd,
+                     double* sumTrP, int32_t* nP, double* time) {
+    const double t0 = now_s();
+    if (!muStart || !muEnd || !nuEnd || !sumTrP || !nP || !time) return FLUE_ENULL;
+    int rc = check_plaq_args(*muStart, *muEnd, *nuEnd);
+    if (rc) return rc;
+    HostField h;
+    if ((rc = load_host_field(NT, NX, NY, NZ, data, h))) return rc;
+    if ((rc = plaquette_count(h.dims, *muStart, *muEnd, *nuEnd, nP))) return rc;
+    DevBuf d;
+    FLUE_TRY(hipMalloc(&d.p, sizeof(double)));
+    if ((rc = flue_plaquette(h.dims, h.soa.p, FLUE_F64, *muStart, *muEnd, *nuEnd, (double*)d.p, nullptr, 0, nullptr)))
+        return rc;
+    if ((rc = fetch_scalar((double*)d.p, sumTrP))) return rc;
+    *time = now_s() - t0;
+    return FLUE_OK;
+}
+
+int genplaquette_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+                   const int32_t* muStart, const int32_t* muEnd, const int32_t* nuEnd, double* sumTrP, int32_t* nP,
+                   double* time) {
+    return plaq_host(NT, NX, NY, NZ, data, muStart, muEnd, nuEnd, sumTrP, nP, time);
+}
+
+int plaquette_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+                const int32_t* muStart, const int32_t* muEnd, const int32_t* nuEnd, double* sumTrP, int32_t* nP,
+                double* time) {
+    return plaq_host(NT, NX, NY, NZ, data, muStart, muEnd, nuEnd, sumTrP, nP, time);
+}
+
+int polyakov_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+               double* sumTrP, int32_t* nP, double* time) {
+    const double t0 = now_s();
+    if (!sumTrP || !nP || !time) return FLUE_ENULL;
+    if (NT && *NT < 3 && *NT >= 1) return FLUE_ENT;
+    HostField h;
+    int rc;
+    if ((rc = load_host_field(NT, NX, NY, NZ, data, h))) return rc;
+    DevBuf d;
+    FLUE_TRY(hipMalloc(&d.p, sizeof(double)));
+    if ((rc = flue_polyakov(h.dims, h.soa.p, FLUE_F64, (double*)d.p, nullptr, 0, nullptr))) return rc;
+    if ((rc = fetch_scalar((double*)d.p, sumTrP))) return rc;
+    *nP = (int32_t)h.L.Vs;
+    *time = now_s() - t0;
+    return FLUE_OK;
+}
+
+static int fmunu_host(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                      const flue_c128* data, const int32_t* mu, const int32_t* nu, flue_c128* out, int variant) {
+    if (!mu || !nu || !out) return FLUE_ENULL;
+    if (*mu < 1 || *mu > 4 || *nu < 1 || *nu > 4) return FLUE_EINVAL;
+    HostField h;
+    int rc;
+    if ((rc = load_host_field(NT, NX, NY, NZ, data, h))) return rc;
+    const size_t n = (size_t)h.L.V * 9;
+    DevBuf d;
+    FLUE_TRY(hipMalloc(&d.p, n * sizeof(flue_c128)));
+    const int64_t V = h.L.V;
+    const int64_t st[6] = {1, h.dims[0], (int64_t)h.dims[0] * h.dims[1], (int64_t)h.dims[0] * h.dims[1] * h.dims[2],
+                           V, 3 * V};
+    if ((rc = flue_fmunu(h.dims, h.soa.p, FLUE_F64, *mu, *nu, variant, d.p, FLUE_OUT_STRIDED, st, nullptr))) return rc;
+    FLUE_TRY(hipMemcpy(out, d.p, n * sizeof(flue_c128), hipMemcpyDeviceToHost));
+    return FLUE_OK;
+}
+
+int cloverloopmunucoord_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                          const flue_c128* data, const int32_t* mu, const int32_t* nu, flue_c128* out) {
+    return fmunu_host(NT, NX, NY, NZ, data, mu, nu, out, 1);
+}
+
+int loop5munucoord_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                     const flue_c128* data, const int32_t* mu, const int32_t* nu, flue_c128* out) {
+    return fmunu_host(NT, NX, NY, NZ, data, mu, nu, out, 0);
+}
+
+int magnetic_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+               double* B) {
+    if (!B) return FLUE_ENULL;
+    HostField h;
+    int rc;
+    if ((rc = load_host_field(NT, NX, NY, NZ, data, h))) return rc;
+    DevBuf d;
+    FLUE_TRY(hipMalloc(&d.p, sizeof(double)));
+    if ((rc = flue_magnetic(h.dims, h.soa.p, FLUE_F64, (double*)d.p, nullptr, 0, nullptr))) return rc;
+    return fetch_scalar((double*)d.p, B);
+}
+
+static int rect_host(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                     const flue_c128* data, int32_t tau, const int32_t* mu, const int32_t* nu, const int32_t* nMu,
+                     const int32_t* nNu, double* loopVal) {
+    if (!mu || !nu || !nMu || !nNu || !loopVal) return FLUE_ENULL;
+    if (*mu < 1 || *mu > 4 || *nu < 1 || *nu > 4 || *nMu < 1 || *nNu < 1) return FLUE_EINVAL;
+    if (2 * (*nMu + *nNu) > 64) return FLUE_EPATH;
+    int32_t path[64];
+    int n = 0;
+    for (int i = 0; i < *nMu; ++i) path[n++] = *mu;
+    for (int i = 0; i < *nNu; ++i) path[n++] = *nu;
+    for (int i = 0; i < *nMu; ++i) path[n++] = -*mu;
+    for (int i = 0; i < *nNu; ++i) path[n++] = -*nu;
+    HostField h;
+    int rc;
+    if ((rc = load_host_field(NT, NX, NY, NZ, data, h))) return rc;
+    if (tau < 0 || tau > h.dims[0]) return FLUE_EINVAL;
+    DevBuf d;
+    FLUE_TRY(hipMalloc(&d.p, sizeof(double)));
+    if ((rc = flue_path_trace(h.dims, h.soa.p, FLUE_F64, path, n, tau, (double*)d.p, nullptr, 0, nullptr))) return rc;
+    double s;
+    if ((rc = fetch_scalar((double*)d.p, &s))) return rc;
+    *loopVal = s / (double)(tau > 0 ? h.L.Vs : h.L.V);
+    return FLUE_OK;
+}
+
+int wmunu_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+            const int32_t* mu, const int32_t* nu, const int32_t* nMu, const int32_t* nNu, double* loopVal) {
+    return rect_host(NT, NX, NY, NZ, data, 0, mu, nu, nMu, nNu, loopVal);
+}
+
+int wmunutau_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+               const int32_t* tau, const int32_t* mu, const int32_t* nu, const int32_t* nMu, const int32_t* nNu,
+               double* loopVal) {
+    if (!tau) return FLUE_ENULL;
+    if (*tau < 1) return FLUE_EINVAL;
+    return rect_host(NT, NX, NY, NZ, data, *tau, mu, nu, nMu, nNu, loopVal);
+}
+
+}  // extern "C"

@@ -1,0 +1,350 @@
+// flue_common.hpp -- device-side building blocks shared by the FLUE HIP kernels.
+//
+// Link storage ("SoA planes"): one real and one imaginary plane per direction and
+// colour element, each a contiguous array over the sites of the lattice:
+//     plane(mu, a, b, ri) = ((mu * 3 + a) * 3 + b) * 2 + ri,   element = plane * V + s
+// with the site index s = x + NX * (y + NY * (z + NZ * t)), so x is fastest and every
+// time slice is one contiguous block of Vs = NX*NY*NZ sites.  Directions use the
+// reference's numbering shifted to 0: mu = 0 is time, 1..3 are x, y, z
+// (src/wloops.F90:21 with mu=1 time).
+//
+// All matrices live in registers as 9 real + 9 imaginary parts, row-major (a*3+b).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flue {
+
+constexpr int kPlanes = 72;  // 4 directions x 9 elements x {re, im}
+
+struct Lat {
+    int n[4];         // extent per direction mu (0 = t, 1 = x, 2 = y, 3 = z)
+    uint32_t st[4];   // site-index stride per direction
+    uint32_t V;       // number of sites
+    uint32_t Vs;      // sites per time slice
+};
+
+inline Lat make_lat(const int32_t dims_tx_yz[4]) {
+    // dims are given in the reference order (NT, NX, NY, NZ)
+    Lat L;
+    for (int i = 0; i < 4; ++i) L.n[i] = dims_tx_yz[i];
+    L.st[1] = 1;
+    L.st[2] = (uint32_t)L.n[1];
+    L.st[3] = (uint32_t)L.n[1] * L.n[2];
+    L.st[0] = (uint32_t)L.n[1] * L.n[2] * L.n[3];
+    L.Vs = L.st[0];
+    L.V = L.Vs * (uint32_t)L.n[0];
+    return L;
+}
+
+template <typename R>
+struct Mat {
+    R r[9], i[9];
+};
+
+// ---------------------------------------------------------------- field access
+
+// Read-only view of a SoA field.  BUF = true reads through one buffer resource per
+// direction (18 planes), so every load of a link shares one 32-bit vector offset and
+// takes its plane offset from a scalar register; it needs 18 * V * sizeof(R) < 2^32
+// bytes.  BUF = false is the plain global-memory form for larger fields.
+template <typename R, bool BUF>
+struct FieldView {
+    using real = R;
+    const R* __restrict__ p;
+    uint32_t V;
+
+    __device__ __forceinline__ const R* plane(int mu, int e, int ri) const {
+        return p + (size_t)(((mu * 9 + e) * 2) + ri) * V;
+    }
+    __device__ __forceinline__ __amdgpu_buffer_rsrc_t dir_rsrc(int mu) const {
+        const char* b = (const char*)p + (size_t)mu * 18u * V * sizeof(R);
+        // dword3 0x00020000: 32-bit data format, no swizzle -- the raw-buffer setting for gfx9 parts
+        return __builtin_amdgcn_make_buffer_rsrc((void*)b, (short)0, (int)(18u * V * (uint32_t)sizeof(R)), 0x00020000);
+    }
+    __device__ __forceinline__ R bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) const {
+        if constexpr (sizeof(R) == 4)
+            return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+        else
+            return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    }
+    // element (mu, e, ri) at site s
+    __device__ __forceinline__ R at(int mu, int e, int ri, uint32_t s) const {
+        return plane(mu, e, ri)[s];
+    }
+};
+
+template <typename R, bool BUF>
+__device__ __forceinline__ Mat<R> load_link(const FieldView<R, BUF>& f, int mu, uint32_t s) {
+    Mat<R> m;
+    if constexpr (BUF) {
+        const __amdgpu_buffer_rsrc_t r = f.dir_rsrc(mu);
+        const uint32_t vo = s * (uint32_t)sizeof(R), pb = f.V * (uint32_t)sizeof(R);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            m.r[e] = f.bld(r, vo, (2 * e) * pb);
+            m.i[e] = f.bld(r, vo, (2 * e + 1) * pb);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            m.r[e] = f.plane(mu, e, 0)[s];
+            m.i[e] = f.plane(mu, e, 1)[s];
+        }
+    }
+    return m;
+}
+
+// U^dagger loaded directly: element (a,b) of U^+ is conj(U_ba)
+template <typename R, bool BUF>
+__device__ __forceinline__ Mat<R> load_link_dag(const FieldView<R, BUF>& f, int mu, uint32_t s) {
+    const Mat<R> u = load_link(f, mu, s);
+    Mat<R> m;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            m.r[a * 3 + b] = u.r[b * 3 + a];
+            m.i[a * 3 + b] = -u.i[b * 3 + a];
+        }
+    return m;
+}
+
+// host side: can the field be read through per-direction buffer resources?
+inline bool fits_buffer(uint32_t V, int is_f32) {
+    return (uint64_t)18 * V * (is_f32 ? 4 : 8) < (1ull << 32);
+}
+
+// host side: call fn with the FieldView type matching the precision and size
+template <typename Fn>
+inline void with_view(uint32_t V, const void* U, int is_f32, Fn&& fn) {
+    if (is_f32) {
+        if (fits_buffer(V, 1))
+            fn(FieldView<float, true>{(const float*)U, V});
+        else
+            fn(FieldView<float, false>{(const float*)U, V});
+    } else {
+        if (fits_buffer(V, 0))
+            fn(FieldView<double, true>{(const double*)U, V});
+        else
+            fn(FieldView<double, false>{(const double*)U, V});
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void store_link(R* __restrict__ p, uint32_t V, int mu, uint32_t s, const Mat<R>& m) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        p[(size_t)((mu * 9 + e) * 2) * V + s] = m.r[e];
+        p[(size_t)((mu * 9 + e) * 2 + 1) * V + s] = m.i[e];
+    }
+}
+
+// ---------------------------------------------------------------- SU(3) algebra
+
+template <typename R>
+__device__ __forceinline__ Mat<R> mul(const Mat<R>& A, const Mat<R>& B) {
+    Mat<R> C;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            R cr = A.r[a * 3] * B.r[b] - A.i[a * 3] * B.i[b];
+            R ci = A.r[a * 3] * B.i[b] + A.i[a * 3] * B.r[b];
+#pragma unroll
+            for (int k = 1; k < 3; ++k) {
+                cr += A.r[a * 3 + k] * B.r[k * 3 + b] - A.i[a * 3 + k] * B.i[k * 3 + b];
+                ci += A.r[a * 3 + k] * B.i[k * 3 + b] + A.i[a * 3 + k] * B.r[k * 3 + b];
+            }
+            C.r[a * 3 + b] = cr;
+            C.i[a * 3 + b] = ci;
+        }
+    return C;
+}
+
+// C = A * B^dagger
+template <typename R>
+__device__ __forceinline__ Mat<R> mul_nd(const Mat<R>& A, const Mat<R>& B) {
+    Mat<R> C;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            R cr = A.r[a * 3] * B.r[b * 3] + A.i[a * 3] * B.i[b * 3];
+            R ci = A.i[a * 3] * B.r[b * 3] - A.r[a * 3] * B.i[b * 3];
+#pragma unroll
+            for (int k = 1; k < 3; ++k) {
+                cr += A.r[a * 3 + k] * B.r[b * 3 + k] + A.i[a * 3 + k] * B.i[b * 3 + k];
+                ci += A.i[a * 3 + k] * B.r[b * 3 + k] - A.r[a * 3 + k] * B.i[b * 3 + k];
+            }
+            C.r[a * 3 + b] = cr;
+            C.i[a * 3 + b] = ci;
+        }
+    return C;
+}
+
+// C = A^dagger * B
+template <typename R>
+__device__ __forceinline__ Mat<R> mul_dn(const Mat<R>& A, const Mat<R>& B) {
+    Mat<R> C;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            R cr = A.r[a] * B.r[b] + A.i[a] * B.i[b];
+            R ci = A.r[a] * B.i[b] - A.i[a] * B.r[b];
+#pragma unroll
+            for (int k = 1; k < 3; ++k) {
+                cr += A.r[k * 3 + a] * B.r[k * 3 + b] + A.i[k * 3 + a] * B.i[k * 3 + b];
+                ci += A.r[k * 3 + a] * B.i[k * 3 + b] - A.i[k * 3 + a] * B.r[k * 3 + b];
+            }
+            C.r[a * 3 + b] = cr;
+            C.i[a * 3 + b] = ci;
+        }
+    return C;
+}
+
+template <typename R>
+__device__ __forceinline__ Mat<R> dag(const Mat<R>& A) {
+    Mat<R> C;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            C.r[a * 3 + b] = A.r[b * 3 + a];
+            C.i[a * 3 + b] = -A.i[b * 3 + a];
+        }
+    return C;
+}
+
+template <typename R>
+__device__ __forceinline__ Mat<R> identity() {
+    Mat<R> C;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        C.r[e] = (e % 4 == 0) ? R(1) : R(0);
+        C.i[e] = R(0);
+    }
+    return C;
+}
+
+// Re Tr(A B^dagger) = sum_ab Re(A_ab conj(B_ab))
+template <typename R>
+__device__ __forceinline__ R retr_nd(const Mat<R>& A, const Mat<R>& B) {
+    R s = A.r[0] * B.r[0] + A.i[0] * B.i[0];
+#pragma unroll
+    for (int e = 1; e < 9; ++e) s += A.r[e] * B.r[e] + A.i[e] * B.i[e];
+    return s;
+}
+
+// Re Tr(A B)
+template <typename R>
+__device__ __forceinline__ R retr_mul(const Mat<R>& A, const Mat<R>& B) {
+    R s = R(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s += A.r[a * 3 + k] * B.r[k * 3 + a] - A.i[a * 3 + k] * B.i[k * 3 + a];
+    return s;
+}
+
+template <typename R>
+__device__ __forceinline__ R retr(const Mat<R>& A) {
+    return A.r[0] + A.r[4] + A.r[8];
+}
+
+// ---------------------------------------------------------------- periodic coordinates
+
+// (c + k) mod n for |k| <= 3 and c in [0, n), branch-free (selects only) so that
+// callers stay a single basic block
+__device__ __forceinline__ int wrapc(int v, int n) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v = (v < 0) ? v + n : v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v = (v >= n) ? v - n : v;
+    return v;
+}
+
+__device__ __forceinline__ void site_coords(const Lat& L, uint32_t s, int c[4]) {
+    uint32_t q = s;
+    c[1] = (int)(q % (uint32_t)L.n[1]);
+    q /= (uint32_t)L.n[1];
+    c[2] = (int)(q % (uint32_t)L.n[2]);
+    q /= (uint32_t)L.n[2];
+    c[3] = (int)(q % (uint32_t)L.n[3]);
+    c[0] = (int)(q / (uint32_t)L.n[3]);
+}
+
+// site reached from s (coords c) by one step along +mu or -mu
+__device__ __forceinline__ uint32_t step_fwd(const Lat& L, uint32_t s, int* c, int mu) {
+    if (c[mu] == L.n[mu] - 1) {
+        c[mu] = 0;
+        return s - (uint32_t)(L.n[mu] - 1) * L.st[mu];
+    }
+    c[mu] += 1;
+    return s + L.st[mu];
+}
+__device__ __forceinline__ uint32_t step_bwd(const Lat& L, uint32_t s, int* c, int mu) {
+    if (c[mu] == 0) {
+        c[mu] = L.n[mu] - 1;
+        return s + (uint32_t)(L.n[mu] - 1) * L.st[mu];
+    }
+    c[mu] -= 1;
+    return s - L.st[mu];
+}
+
+// Hardware hands consecutive workgroups to the 8 XCDs in turn; give each XCD a
+// contiguous range of logical blocks so that its private L2 sees neighbouring sites.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
+    const uint32_t X = 8;
+    uint32_t q = nblocks / X, r = nblocks % X;
+    uint32_t x = bid % X, k = bid / X;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// ---------------------------------------------------------------- compensated sums
+
+struct DD {
+    double hi, lo;
+};
+
+__device__ __forceinline__ DD two_sum(double a, double b) {
+    double s = a + b;
+    double bb = s - a;
+    double e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    DD s = two_sum(a.hi, b.hi);
+    double lo = s.lo + a.lo + b.lo;
+    double hi = s.hi + lo;
+    return {hi, lo - (hi - s.hi)};
+}
+
+__device__ __forceinline__ DD dd_shfl_xor(DD v, int m) {
+    return {__shfl_xor(v.hi, m), __shfl_xor(v.lo, m)};
+}
+
+// Deterministic block sum of a compensated pair: fixed butterfly inside each
+// wavefront, then the waves in index order.  Result valid in thread 0.
+template <int BS>
+__device__ __forceinline__ DD block_sum(DD v) {
+    __shared__ double sh_hi[BS / 64], sh_lo[BS / 64];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = dd_add(v, dd_shfl_xor(v, m));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh_hi[w] = v.hi;
+        sh_lo[w] = v.lo;
+    }
+    __syncthreads();
+    DD r = {0.0, 0.0};
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < BS / 64; ++k) r = dd_add(r, DD{sh_hi[k], sh_lo[k]});
+    }
+    return r;
+}
+
+}  // namespace flue

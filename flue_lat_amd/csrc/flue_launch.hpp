@@ -1,0 +1,43 @@
+// flue_launch.hpp -- host-side launchers implemented by the kernel files and used by
+// the C ABI (capi.hip).  All launchers are asynchronous on the given stream and
+// return a hipError_t from the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "flue_common.hpp"
+
+namespace flue {
+
+// field layout conversion: complex input/output with element strides (in complex
+// elements) for the index order (t, x, y, z, mu, a, b)
+hipError_t launch_import(const Lat& L, const void* src, int src_is_f32, const int64_t strides[7], void* dst,
+                         int dst_is_f32, hipStream_t st);
+hipError_t launch_export(const Lat& L, const void* src, int src_is_f32, void* dst, int dst_is_f32,
+                         const int64_t strides[7], hipStream_t st);
+hipError_t launch_random_field(const Lat& L, uint64_t seed, double eps, void* dst, int is_f32, hipStream_t st);
+hipError_t launch_convert(const Lat& L, const void* src, int src_is_f32, void* dst, int dst_is_f32, hipStream_t st);
+
+// reductions: each writes one compensated partial (hi, lo) per block to `partials`,
+// then launch_finalize sums them in a fixed order into out[0] (double, device)
+uint32_t plaquette_blocks(const Lat& L);
+hipError_t launch_plaquette(const Lat& L, const void* U, int is_f32, int mu0, int mu1, int nu1, double* partials,
+                            hipStream_t st);
+uint32_t polyakov_blocks(const Lat& L);
+hipError_t launch_polyakov(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st);
+uint32_t path_blocks(const Lat& L, int tslice);
+hipError_t launch_path_trace(const Lat& L, const void* U, int is_f32, const int32_t* path, int npath, int tslice,
+                             double* partials, hipStream_t st);
+hipError_t launch_generic_path(const Lat& L, const void* U, int is_f32, const int32_t base[4], const int32_t* path,
+                               int npath, double* out18, hipStream_t st);
+hipError_t launch_finalize(const double* partials, uint32_t n, double* out, double scale, hipStream_t st);
+
+// 5-loop (or 1x1 clover) field strength for one plane; out_layout 0 = complex128 with
+// strides (t, x, y, z, a, b) in complex elements, 1 = SoA planes (9 elem x re/im) in the
+// field precision
+hipError_t launch_fmunu(const Lat& L, const void* U, int is_f32, int mu, int nu, int clover_only, void* out,
+                        int out_layout, const int64_t strides[6], hipStream_t st);
+uint32_t magnetic_blocks(const Lat& L, int is_f32);
+hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st);
+
+}  // namespace flue

@@ -1,0 +1,148 @@
+// k_field.hip -- moving gauge fields between the reference's complex layout and the
+// SoA planes, precision conversion, and the on-device link generator.
+#include "flue_launch.hpp"
+#include "flue_su3x.hpp"
+
+namespace flue {
+
+namespace {
+
+template <typename Rin, typename Rout>
+__global__ void __launch_bounds__(256) import_kernel(Lat L, const Rin* __restrict__ src, int64_t st_t, int64_t st_x,
+                                                     int64_t st_y, int64_t st_z, int64_t st_mu, int64_t st_a,
+                                                     int64_t st_b, Rout* __restrict__ dst) {
+    // one thread per (site, mu); threads run along the reference's t-fastest order so
+    // that reads of a Fortran-ordered source are contiguous
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (uint64_t)L.V * 4) return;
+    const uint32_t mu = (uint32_t)(gid / L.V);
+    uint32_t q = (uint32_t)(gid - (uint64_t)mu * L.V);
+    const int t = (int)(q % (uint32_t)L.n[0]);
+    q /= (uint32_t)L.n[0];
+    const int x = (int)(q % (uint32_t)L.n[1]);
+    q /= (uint32_t)L.n[1];
+    const int y = (int)(q % (uint32_t)L.n[2]);
+    const int z = (int)(q / (uint32_t)L.n[2]);
+    const int64_t base = t * st_t + x * st_x + y * st_y + z * st_z + (int64_t)mu * st_mu;
+    const uint32_t s = x + L.st[2] * y + L.st[3] * z + L.st[0] * t;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int64_t off = 2 * (base + a * st_a + b * st_b);
+            dst[(size_t)((mu * 9 + a * 3 + b) * 2) * L.V + s] = (Rout)src[off];
+            dst[(size_t)((mu * 9 + a * 3 + b) * 2 + 1) * L.V + s] = (Rout)src[off + 1];
+        }
+}
+
+template <typename Rin, typename Rout>
+__global__ void __launch_bounds__(256) export_kernel(Lat L, const Rin* __restrict__ src, Rout* __restrict__ dst,
+                                                     int64_t st_t, int64_t st_x, int64_t st_y, int64_t st_z,
+                                                     int64_t st_mu, int64_t st_a, int64_t st_b) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (uint64_t)L.V * 4) return;
+    const uint32_t mu = (uint32_t)(gid / L.V);
+    uint32_t q = (uint32_t)(gid - (uint64_t)mu * L.V);
+    const int t = (int)(q % (uint32_t)L.n[0]);
+    q /= (uint32_t)L.n[0];
+    const int x = (int)(q % (uint32_t)L.n[1]);
+    q /= (uint32_t)L.n[1];
+    const int y = (int)(q % (uint32_t)L.n[2]);
+    const int z = (int)(q / (uint32_t)L.n[2]);
+    const int64_t base = t * st_t + x * st_x + y * st_y + z * st_z + (int64_t)mu * st_mu;
+    const uint32_t s = x + L.st[2] * y + L.st[3] * z + L.st[0] * t;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int64_t off = 2 * (base + a * st_a + b * st_b);
+            dst[off] = (Rout)src[(size_t)((mu * 9 + a * 3 + b) * 2) * L.V + s];
+            dst[off + 1] = (Rout)src[(size_t)((mu * 9 + a * 3 + b) * 2 + 1) * L.V + s];
+        }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) random_kernel(Lat L, uint64_t seed, double eps, R* __restrict__ dst) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (uint64_t)L.V * 4) return;
+    const uint32_t mu = (uint32_t)(gid / L.V);
+    const uint32_t s = (uint32_t)(gid - (uint64_t)mu * L.V);
+    int c[4];
+    site_coords(L, s, c);
+    const uint64_t sref = (uint64_t)c[0] + (uint64_t)L.n[0] * (c[1] + (uint64_t)L.n[1] * (c[2] + (uint64_t)L.n[2] * c[3]));
+    const Mat<double> U = random_link(seed, sref + (uint64_t)L.V * mu, eps);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        dst[(size_t)((mu * 9 + e) * 2) * L.V + s] = (R)U.r[e];
+        dst[(size_t)((mu * 9 + e) * 2 + 1) * L.V + s] = (R)U.i[e];
+    }
+}
+
+template <typename Rin, typename Rout>
+__global__ void __launch_bounds__(256) convert_kernel(size_t n, const Rin* __restrict__ src, Rout* __restrict__ dst) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < n) dst[gid] = (Rout)src[gid];
+}
+
+inline uint32_t grid_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t launch_import(const Lat& L, const void* src, int src_is_f32, const int64_t s[7], void* dst,
+                         int dst_is_f32, hipStream_t st) {
+    const uint32_t g = grid_for((uint64_t)L.V * 4);
+    if (src_is_f32 && dst_is_f32)
+        import_kernel<float, float><<<g, 256, 0, st>>>(L, (const float*)src, s[0], s[1], s[2], s[3], s[4], s[5], s[6],
+                                                       (float*)dst);
+    else if (src_is_f32)
+        import_kernel<float, double><<<g, 256, 0, st>>>(L, (const float*)src, s[0], s[1], s[2], s[3], s[4], s[5],
+                                                        s[6], (double*)dst);
+    else if (dst_is_f32)
+        import_kernel<double, float><<<g, 256, 0, st>>>(L, (const double*)src, s[0], s[1], s[2], s[3], s[4], s[5],
+                                                        s[6], (float*)dst);
+    else
+        import_kernel<double, double><<<g, 256, 0, st>>>(L, (const double*)src, s[0], s[1], s[2], s[3], s[4], s[5],
+                                                         s[6], (double*)dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_export(const Lat& L, const void* src, int src_is_f32, void* dst, int dst_is_f32,
+                         const int64_t s[7], hipStream_t st) {
+    const uint32_t g = grid_for((uint64_t)L.V * 4);
+    if (src_is_f32 && dst_is_f32)
+        export_kernel<float, float><<<g, 256, 0, st>>>(L, (const float*)src, (float*)dst, s[0], s[1], s[2], s[3], s[4],
+                                                       s[5], s[6]);
+    else if (src_is_f32)
+        export_kernel<float, double><<<g, 256, 0, st>>>(L, (const float*)src, (double*)dst, s[0], s[1], s[2], s[3],
+                                                        s[4], s[5], s[6]);
+    else if (dst_is_f32)
+        export_kernel<double, float><<<g, 256, 0, st>>>(L, (const double*)src, (float*)dst, s[0], s[1], s[2], s[3],
+                                                        s[4], s[5], s[6]);
+    else
+        export_kernel<double, double><<<g, 256, 0, st>>>(L, (const double*)src, (double*)dst, s[0], s[1], s[2], s[3],
+                                                         s[4], s[5], s[6]);
+    return hipGetLastError();
+}
+
+hipError_t launch_random_field(const Lat& L, uint64_t seed, double eps, void* dst, int is_f32, hipStream_t st) {
+    const uint32_t g = grid_for((uint64_t)L.V * 4);
+    if (is_f32)
+        random_kernel<float><<<g, 256, 0, st>>>(L, seed, eps, (float*)dst);
+    else
+        random_kernel<double><<<g, 256, 0, st>>>(L, seed, eps, (double*)dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const Lat& L, const void* src, int src_is_f32, void* dst, int dst_is_f32, hipStream_t st) {
+    const size_t n = (size_t)L.V * kPlanes;
+    const uint32_t g = grid_for(n);
+    if (src_is_f32 == dst_is_f32)
+        return hipMemcpyAsync(dst, src, n * (src_is_f32 ? 4 : 8), hipMemcpyDeviceToDevice, st);
+    if (src_is_f32)
+        convert_kernel<float, double><<<g, 256, 0, st>>>(n, (const float*)src, (double*)dst);
+    else
+        convert_kernel<double, float><<<g, 256, 0, st>>>(n, (const double*)src, (float*)dst);
+    return hipGetLastError();
+}
+
+}  // namespace flue

@@ -1,0 +1,223 @@
+// k_loops.hip -- plaquette family, Polyakov loop, generic-path products and the
+// deterministic second pass of every lattice sum.
+#include "flue_launch.hpp"
+
+namespace flue {
+
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kMaxPath = 64;
+
+struct PathArg {
+    int n;
+    int8_t d[kMaxPath];  // signed directions, 1-based as in the reference (mu=1 is time)
+};
+
+__device__ __forceinline__ uint32_t site_of(const Lat& L, const int c[4]) {
+    return (uint32_t)c[1] + L.st[2] * (uint32_t)c[2] + L.st[3] * (uint32_t)c[3] + L.st[0] * (uint32_t)c[0];
+}
+
+__device__ __forceinline__ void write_partial(double* partials, uint32_t blk, DD r) {
+    if (threadIdx.x == 0) {
+        partials[2 * blk] = r.hi;
+        partials[2 * blk + 1] = r.lo;
+    }
+}
+
+// sum over sites and planes mu0 <= mu <= mu1, mu < nu <= nu1 of
+// ReTr[(U_mu(x) U_nu(x+mu)) (U_nu(x) U_mu(x+nu))^+]   (src/wloops.F90:947-1006)
+template <typename FV>
+__global__ void __launch_bounds__(kBS) plaquette_kernel(Lat L, FV f, int mu0, int mu1, int nu1,
+                                                        double* __restrict__ partials) {
+    using R = typename FV::real;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = blk * kBS + threadIdx.x;
+    double acc = 0.0;
+    if (s < L.V) {
+        int c[4];
+        site_coords(L, s, c);
+        for (int mu = mu0; mu <= mu1; ++mu) {
+            const Mat<R> Umu = load_link(f, mu, s);
+            int cm[4] = {c[0], c[1], c[2], c[3]};
+            const uint32_t sm = step_fwd(L, s, cm, mu);
+            for (int nu = mu + 1; nu <= nu1; ++nu) {
+                int cn[4] = {c[0], c[1], c[2], c[3]};
+                const uint32_t sn = step_fwd(L, s, cn, nu);
+                const Mat<R> A = mul(Umu, load_link(f, nu, sm));
+                const Mat<R> B = mul(load_link(f, nu, s), load_link(f, mu, sn));
+                acc += (double)retr_nd(A, B);
+            }
+        }
+    }
+    write_partial(partials, blk, block_sum<kBS>(DD{acc, 0.0}));
+}
+
+// Polyakov loop: ordered product of the time links over t = 0..NT-1 at each spatial
+// site (src/wloops.F90:1043-1063).  A block holds 64 spatial sites x 4 time segments;
+// segment products are combined left to right in shared memory.
+template <typename FV>
+__global__ void __launch_bounds__(kBS) polyakov_kernel(Lat L, FV f, double* __restrict__ partials) {
+    using R = typename FV::real;
+    constexpr int NSEG = kBS / 64;
+    __shared__ R sh[NSEG][18][64];
+    const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sp = blk * 64 + lane;
+    const bool live = sp < L.Vs;
+    const int NT = L.n[0];
+    const int t0 = (seg * NT) / NSEG, t1 = ((seg + 1) * NT) / NSEG;
+    Mat<R> P = identity<R>();
+    if (live) {
+        for (int t = t0; t < t1; ++t) {
+            const Mat<R> U = load_link(f, 0, sp + L.Vs * (uint32_t)t);
+            P = (t == t0) ? U : mul(P, U);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        sh[seg][e][lane] = P.r[e];
+        sh[seg][9 + e][lane] = P.i[e];
+    }
+    __syncthreads();
+    double val = 0.0;
+    if (seg == 0 && live) {
+        Mat<R> T = P;
+        for (int k = 1; k < NSEG; ++k) {
+            Mat<R> Q;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                Q.r[e] = sh[k][e][lane];
+                Q.i[e] = sh[k][9 + e][lane];
+            }
+            T = mul(T, Q);
+        }
+        val = (double)retr(T);
+    }
+    __syncthreads();
+    write_partial(partials, blk, block_sum<kBS>(DD{val, 0.0}));
+}
+
+// product along a signed path starting at the site s with coordinates c (modified):
+// +mu multiplies U_mu(x) then steps, -mu steps back then multiplies U_mu(x)^+
+// (src/wloops.F90:36-55)
+template <typename FV>
+__device__ __forceinline__ Mat<typename FV::real> walk_path(const Lat& L, const FV& f, const PathArg& p, uint32_t s,
+                                                            int c[4]) {
+    using R = typename FV::real;
+    Mat<R> M = identity<R>();
+    for (int k = 0; k < p.n; ++k) {
+        const int d = p.d[k];
+        const int ax = (d > 0 ? d : -d) - 1;
+        Mat<R> U;
+        if (d > 0) {
+            U = load_link(f, ax, s);
+            s = step_fwd(L, s, c, ax);
+        } else {
+            s = step_bwd(L, s, c, ax);
+            U = load_link_dag(f, ax, s);
+        }
+        M = (k == 0) ? U : mul(M, U);
+    }
+    return M;
+}
+
+template <typename FV>
+__global__ void __launch_bounds__(kBS) path_trace_kernel(Lat L, FV f, PathArg p, int tslice,
+                                                         double* __restrict__ partials) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t idx = blk * kBS + threadIdx.x;
+    const uint32_t n = tslice >= 0 ? L.Vs : L.V;
+    double acc = 0.0;
+    if (idx < n) {
+        const uint32_t s = tslice >= 0 ? idx + L.Vs * (uint32_t)tslice : idx;
+        int c[4];
+        site_coords(L, s, c);
+        acc = (double)retr(walk_path(L, f, p, s, c));
+    }
+    write_partial(partials, blk, block_sum<kBS>(DD{acc, 0.0}));
+}
+
+// one path product at one base point, written transposed as a Fortran-ordered
+// complex128 3x3 (src/wloops.F90:56)
+template <typename FV>
+__global__ void generic_path_kernel(Lat L, FV f, PathArg p, int t, int x, int y, int z, double* __restrict__ out) {
+    using R = typename FV::real;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int c[4] = {t, x, y, z};
+    const uint32_t s = site_of(L, c);
+    const Mat<R> M = walk_path(L, f, p, s, c);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            out[2 * (a + 3 * b)] = (double)M.r[b * 3 + a];
+            out[2 * (a + 3 * b) + 1] = (double)M.i[b * 3 + a];
+        }
+}
+
+// fixed-order compensated sum of per-block partials
+__global__ void __launch_bounds__(kBS) finalize_kernel(const double* __restrict__ partials, uint32_t n,
+                                                       double* __restrict__ out, double scale) {
+    const uint32_t lo = (uint32_t)(((uint64_t)n * threadIdx.x) / kBS);
+    const uint32_t hi = (uint32_t)(((uint64_t)n * (threadIdx.x + 1)) / kBS);
+    DD acc = {0.0, 0.0};
+    for (uint32_t i = lo; i < hi; ++i) acc = dd_add(acc, DD{partials[2 * i], partials[2 * i + 1]});
+    DD r = block_sum<kBS>(acc);
+    if (threadIdx.x == 0) out[0] = (r.hi + r.lo) * scale;
+}
+
+PathArg make_path(const int32_t* path, int npath) {
+    PathArg p;
+    p.n = npath;
+    for (int i = 0; i < kMaxPath; ++i) p.d[i] = (int8_t)(i < npath ? path[i] : 0);
+    return p;
+}
+
+}  // namespace
+
+uint32_t plaquette_blocks(const Lat& L) { return (L.V + kBS - 1) / kBS; }
+
+hipError_t launch_plaquette(const Lat& L, const void* U, int is_f32, int mu0, int mu1, int nu1, double* partials,
+                            hipStream_t st) {
+    const uint32_t g = plaquette_blocks(L);
+    with_view(L.V, U, is_f32, [&](auto f) {
+        plaquette_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, mu0, mu1, nu1, partials);
+    });
+    return hipGetLastError();
+}
+
+uint32_t polyakov_blocks(const Lat& L) { return (L.Vs + 63) / 64; }
+
+hipError_t launch_polyakov(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st) {
+    const uint32_t g = polyakov_blocks(L);
+    with_view(L.V, U, is_f32, [&](auto f) { polyakov_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, partials); });
+    return hipGetLastError();
+}
+
+uint32_t path_blocks(const Lat& L, int tslice) { return ((tslice >= 0 ? L.Vs : L.V) + kBS - 1) / kBS; }
+
+hipError_t launch_path_trace(const Lat& L, const void* U, int is_f32, const int32_t* path, int npath, int tslice,
+                             double* partials, hipStream_t st) {
+    if (npath < 1 || npath > kMaxPath) return hipErrorInvalidValue;
+    const PathArg p = make_path(path, npath);
+    const uint32_t g = path_blocks(L, tslice);
+    with_view(L.V, U, is_f32,
+              [&](auto f) { path_trace_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, p, tslice, partials); });
+    return hipGetLastError();
+}
+
+hipError_t launch_generic_path(const Lat& L, const void* U, int is_f32, const int32_t base[4], const int32_t* path,
+                               int npath, double* out18, hipStream_t st) {
+    if (npath < 0 || npath > kMaxPath) return hipErrorInvalidValue;
+    const PathArg p = make_path(path, npath);
+    with_view(L.V, U, is_f32, [&](auto f) {
+        generic_path_kernel<decltype(f)><<<1, 64, 0, st>>>(L, f, p, base[0], base[1], base[2], base[3], out18);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double* partials, uint32_t n, double* out, double scale, hipStream_t st) {
+    finalize_kernel<<<1, kBS, 0, st>>>(partials, n, out, scale);
+    return hipGetLastError();
+}
+
+}  // namespace flue

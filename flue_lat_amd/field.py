@@ -1,0 +1,198 @@
+"""Device-resident gauge fields and the device entry points of libflue_hip.so.
+
+A :class:`GaugeField` owns one torch tensor holding the library's SoA layout (72
+planes of V reals: direction mu, colour a, b, real/imaginary; site index
+x + NX*(y + NY*(z + NZ*t))) in FP64 or FP32.  Its methods call the C ABI on the
+current torch stream and keep the results on the device until asked for.
+
+Directions and coordinates follow the reference (src/wloops.F90): mu = 1 is time,
+2..4 are x, y, z; coordinates are 1-based (t, x, y, z).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import FLUE_C64, FLUE_C128, FLUE_F32, FLUE_F64, FLUE_OUT_SOA, FLUE_OUT_STRIDED, check
+
+_PREC = {"fp64": FLUE_F64, "f64": FLUE_F64, "double": FLUE_F64, torch.float64: FLUE_F64,
+         "fp32": FLUE_F32, "f32": FLUE_F32, "float": FLUE_F32, torch.float32: FLUE_F32}
+
+
+def _prec(p):
+    try:
+        return _PREC[p]
+    except KeyError:
+        raise ValueError(f"precision must be 'fp64' or 'fp32', got {p!r}") from None
+
+
+def _i32(*vals):
+    return (ctypes.c_int32 * len(vals))(*vals)
+
+
+def _i64(vals):
+    return (ctypes.c_int64 * len(vals))(*vals)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("flue_lat_amd device path needs a visible GPU")
+
+
+class GaugeField:
+    """SU(3) gauge field on the GPU in the SoA layout.
+
+    dims = (NT, NX, NY, NZ) as in the reference's data(NT,NX,NY,NZ,4,3,3).
+    """
+
+    def __init__(self, dims, precision="fp64", device=None, _data=None):
+        require_gpu()
+        self.dims = tuple(int(d) for d in dims)
+        if len(self.dims) != 4 or min(self.dims) < 1:
+            raise ValueError(f"dims must be 4 positive extents (NT, NX, NY, NZ), got {dims}")
+        self.prec = _prec(precision)
+        self.dtype = torch.float32 if self.prec == FLUE_F32 else torch.float64
+        self.V = int(np.prod(self.dims))
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if _data is None:
+            _data = torch.empty(72 * self.V, dtype=self.dtype, device=self.device)
+        self.data = _data
+        self._cdims = _i32(*self.dims)
+        self._ws = None
+
+    # ------------------------------------------------------------ construction
+    @property
+    def precision(self):
+        return "fp32" if self.prec == FLUE_F32 else "fp64"
+
+    def workspace(self):
+        if self._ws is None:
+            n = _lib.lib().flue_workspace_bytes(self._cdims)
+            self._ws = torch.empty(n, dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    @classmethod
+    def random(cls, dims, seed, eps, precision="fp64", device=None):
+        """Seeded synthetic field U = fix(exp(i eps sum_a c_a lambda_a/2)), generated on the GPU."""
+        f = cls(dims, precision, device)
+        with torch.cuda.device(f.device):
+            check(_lib.lib().flue_random_field(f._cdims, ctypes.c_uint64(seed), float(eps), _ptr(f.data), f.prec,
+                                               _stream()), "flue_random_field")
+        return f
+
+    @classmethod
+    def from_complex(cls, U, precision="fp64", device=None):
+        """From a (NT,NX,NY,NZ,4,3,3) complex array: numpy (any order) or a torch tensor
+        (any strides, complex128 or complex64, CPU or GPU)."""
+        if isinstance(U, np.ndarray):
+            U = torch.from_numpy(np.asarray(U))
+        if U.dim() != 7 or tuple(U.shape[4:]) != (4, 3, 3):
+            raise ValueError("gauge field must have shape (NT,NX,NY,NZ,4,3,3)")
+        if U.dtype not in (torch.complex128, torch.complex64):
+            U = U.to(torch.complex128)
+        f = cls(U.shape[:4], precision, device)
+        src = U.to(f.device) if U.device != f.device else U
+        with torch.cuda.device(f.device):
+            src_type = FLUE_C64 if src.dtype == torch.complex64 else FLUE_C128
+            check(_lib.lib().flue_import_field(f._cdims, _ptr(src), src_type, _i64(src.stride()), _ptr(f.data), f.prec,
+                                               _stream()), "flue_import_field")
+            if src is not U:
+                torch.cuda.current_stream().synchronize()
+        return f
+
+    def to_complex(self, dtype=torch.complex128):
+        """Export as a (NT,NX,NY,NZ,4,3,3) complex tensor with the reference's Fortran strides."""
+        NT, NX, NY, NZ = self.dims
+        buf = torch.empty((3, 3, 4, NZ, NY, NX, NT), dtype=dtype, device=self.device)
+        out = buf.permute(6, 5, 4, 3, 2, 1, 0)
+        with torch.cuda.device(self.device):
+            dst_type = FLUE_C64 if dtype == torch.complex64 else FLUE_C128
+            check(_lib.lib().flue_export_field(self._cdims, _ptr(self.data), self.prec, _ptr(out), dst_type,
+                                               _i64(out.stride()), _stream()), "flue_export_field")
+        return out
+
+    def astype(self, precision):
+        p = _prec(precision)
+        g = GaugeField(self.dims, "fp32" if p == FLUE_F32 else "fp64", self.device)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_convert_field(self._cdims, _ptr(self.data), self.prec, _ptr(g.data), g.prec,
+                                                _stream()), "flue_convert_field")
+        return g
+
+    # ------------------------------------------------------------ observables
+    def _scalar(self):
+        return torch.empty(1, dtype=torch.float64, device=self.device)
+
+    def plaquette(self, muStart=1, muEnd=4, nuEnd=4, sync=True):
+        """(sum of ReTr P, number of plaquettes) as in genPlaquette / plaquette (src/wloops.F90:59-127, 915-1017)."""
+        out = self._scalar()
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_plaquette(self._cdims, _ptr(self.data), self.prec, muStart, muEnd, nuEnd, _ptr(out),
+                                            _ptr(ws), ws.numel(), _stream()), "flue_plaquette")
+        nP = sum(max(0, nuEnd - mu) for mu in range(muStart, muEnd + 1)) * self.V
+        return (out.item() if sync else out), nP
+
+    def polyakov(self, sync=True):
+        """(sum_xyz ReTr prod_t U_t, NX*NY*NZ) as in polyakov (src/wloops.F90:1019-1069)."""
+        out = self._scalar()
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_polyakov(self._cdims, _ptr(self.data), self.prec, _ptr(out), _ptr(ws), ws.numel(),
+                                           _stream()), "flue_polyakov")
+        return (out.item() if sync else out), self.V // self.dims[0]
+
+    def path_trace(self, path, tau=0, sync=True):
+        """Sum of ReTr genericPath over all sites (tau=0) or over time slice tau (1-based)."""
+        p = _i32(*[int(v) for v in path])
+        out = self._scalar()
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_path_trace(self._cdims, _ptr(self.data), self.prec, p, len(path), int(tau), _ptr(out),
+                                             _ptr(ws), ws.numel(), _stream()), "flue_path_trace")
+        return out.item() if sync else out
+
+    def wilson_loop(self, mu, nu, nMu, nNu, tau=0):
+        """wmunu (tau=0) or wmunutau: mean ReTr of the nMu x nNu loop (src/wloops.F90:567-791)."""
+        path = [mu] * nMu + [nu] * nNu + [-mu] * nMu + [-nu] * nNu
+        s = self.path_trace(path, tau)
+        return s / (self.V // self.dims[0] if tau else self.V)
+
+    def generic_path(self, coordBase, path):
+        """genericPath at one 1-based base point: the transposed ordered product (src/wloops.F90:20-57)."""
+        out = torch.empty(18, dtype=torch.float64, device=self.device)
+        p = _i32(*[int(v) for v in path])
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_generic_path(self._cdims, _ptr(self.data), self.prec, _i32(*coordBase), p, len(path),
+                                               _ptr(out), _stream()), "flue_generic_path")
+        v = out.cpu().numpy()
+        return (v[0::2] + 1j * v[1::2]).reshape(3, 3, order="F")
+
+    def fmunu(self, mu, nu, clover=False):
+        """Per-site F for plane (mu, nu): Loop5MuNuCoord, or cloverLoopMuNuCoord with clover=True.
+        Returns a complex128 tensor of shape (NT,NX,NY,NZ,3,3) with Fortran strides."""
+        NT, NX, NY, NZ = self.dims
+        buf = torch.empty((3, 3, NZ, NY, NX, NT), dtype=torch.complex128, device=self.device)
+        out = buf.permute(5, 4, 3, 2, 1, 0)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_fmunu(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu), 1 if clover else 0,
+                                        _ptr(out), FLUE_OUT_STRIDED, _i64(out.stride()), _stream()), "flue_fmunu")
+        return out
+
+    def magnetic(self, sync=True):
+        """B = -(1/V) sum_x sum_i ReTr F_i^2 over the spatial planes (src/wloops.F90:479-544)."""
+        out = self._scalar()
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_magnetic(self._cdims, _ptr(self.data), self.prec, _ptr(out), _ptr(ws), ws.numel(),
+                                           _stream()), "flue_magnetic")
+        return out.item() if sync else out

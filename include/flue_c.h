@@ -1,0 +1,192 @@
+/*
+ * flue_c.h -- C ABI of libflue_hip.so, the MI355X implementation of FLUE's
+ * gluonic-observable path (plaquette family, Polyakov loop, generic path
+ * products, Wilson loops, 5-loop F_munu, magnetic B^2, their gradients and the
+ * Wilson flow).
+ *
+ * Two groups of entry points:
+ *
+ * 1. Host drop-ins named like the reference's FFI layer (c_wrapper/FLUE_c.f90,
+ *    wrapped by f2py as FLUE.compiled.flue_c with --lower).  They follow its
+ *    conventions: every argument by reference, lattice dims first, arrays in
+ *    Fortran order as complex128 (double _Complex) host memory
+ *    data(NT,NX,NY,NZ,4,3,3), scalars next, results last.  Where the
+ *    reference's Fortran procedure stops or has undefined behaviour, these
+ *    return a nonzero status instead.  The GPU does the work: the field is
+ *    copied to the device, re-laid out and measured there.
+ *
+ * 2. Device entry points (flue_*) on device-resident fields in the library's
+ *    SoA layout: 72 planes (direction mu, colour a, b, re/im) of V reals, site
+ *    s = x + NX*(y + NY*(z + NZ*t)), plane ((mu*3 + a)*3 + b)*2 + ri, FP64 or
+ *    FP32.  They are asynchronous on the given hipStream_t (NULL = default
+ *    stream), write results to device memory and never synchronise.  Lattice
+ *    sums are deterministic (fixed reduction order) and use compensated FP64
+ *    partial sums; `ws` is a device workspace of at least
+ *    flue_workspace_bytes(dims) bytes (NULL = a buffer the library keeps per
+ *    device, grown on demand -- do not pass NULL under stream capture).
+ *
+ * Directions follow the reference: mu = 1 is time, 2..4 are x, y, z
+ * (1-based, as in src/wloops.F90), and dims are {NT, NX, NY, NZ}.
+ * Every function returns FLUE_OK (0) or one of the FLUE_E* codes.
+ */
+#ifndef FLUE_C_H
+#define FLUE_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+typedef struct {
+    double re, im;
+} flue_c128;
+#else
+typedef double _Complex flue_c128;
+#endif
+
+enum {
+    FLUE_OK = 0,
+    FLUE_EINVAL = 1,   /* argument value out of range */
+    FLUE_ENULL = 2,    /* required pointer is NULL */
+    FLUE_EPATH = 3,    /* path entry 0 or |mu| > 4, or path too long */
+    FLUE_ENT = 4,      /* Polyakov loop needs NT >= 3 */
+    FLUE_EDIM = 5,     /* lattice extent < 1 or volume too large */
+    FLUE_EHIP = 6,     /* HIP runtime error */
+    FLUE_ENOMEM = 7,   /* device allocation failed */
+    FLUE_ENOTIMPL = 8, /* outside this library's scope */
+    FLUE_ENODEV = 9,   /* no GPU visible */
+    FLUE_EWS = 10      /* workspace too small */
+};
+
+enum { FLUE_F64 = 0, FLUE_F32 = 1 };        /* link precision of a SoA field */
+enum { FLUE_C128 = 0, FLUE_C64 = 1 };       /* element type of a complex array */
+enum { FLUE_OUT_STRIDED = 0, FLUE_OUT_SOA = 1 };
+
+const char* flue_strerror(int status);
+int flue_version(int32_t* major, int32_t* minor, int32_t* patch);
+
+/* ------------------------------------------------------------------ */
+/* 1. host drop-ins (c_wrapper/FLUE_c.f90 conventions)                  */
+/* ------------------------------------------------------------------ */
+
+/* MultiplyMatMat_c, c_wrapper/FLUE_c.f90:31-36 (src/SU3MatrixOps.f90:19-37): MM = left*right */
+int multiplymatmat_c(flue_c128 MM[9], const flue_c128 left[9], const flue_c128 right[9]);
+
+/* get_qhat_c, c_wrapper/FLUE_c.f90:49-56 (src/gluonAna/momentum.f90:8-24):
+ * qhat = (2 pi / a) coord / shape; a == NULL means a = 1 (the reference divides by
+ * the absent argument, momentum.f90:23) */
+int get_qhat_c(const double coord[4], const int32_t shape[4], double qhat[4], const double* a);
+
+/* propagator entries of c_wrapper/FLUE_c.f90:9-29,38-47,58-71 are outside this
+ * library's scope and return FLUE_ENOTIMPL */
+int scalargluonprop_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, const double* xi, double* D);
+int calc_mom_space_scalard_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, flue_c128* D,
+                             const int32_t* mu_start, const double* xi);
+int q_average_c(const int32_t* NQ, const double* q, const double* D, const int32_t* q_slices, double* DOUT,
+                double* QOUT, int32_t* Qcount);
+int cone_cut_c(const int32_t* NQ, const int32_t* radius, const double* Q, const double* D, const double* D4,
+               const int32_t* NT, const int32_t* NS, const double* angleIN, const double* xiIN,
+               const double* IRCutIN, const double* IRRadiusIN, double* QOUT, double* Dout, double* D4out,
+               int32_t* qcount);
+
+/* periodCoord, src/wloops.F90:546-565: single-step periodic wrap of a 1-based
+ * coordinate; dataShape is the 7-entry SHAPE(data) */
+int periodcoord_c(const int32_t coord[4], const int32_t dataShape[7], int32_t out[4]);
+
+/* genericPath, src/wloops.F90:20-57: ordered product along path(1:npath) from the
+ * 1-based coordBase, returned transposed as a Fortran 3x3 */
+int genericpath_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                  const flue_c128* data, const int32_t coordBase[4], const int32_t* npath, const int32_t* path,
+                  flue_c128 out[9]);
+
+/* genPlaquette, src/wloops.F90:59-127 */
+int genplaquette_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                   const flue_c128* data, const int32_t* muStart, const int32_t* muEnd, const int32_t* nuEnd,
+                   double* sumTrP, int32_t* nP, double* time);
+
+/* plaquette, src/wloops.F90:915-1017 (assumed-shape there; dims passed here) */
+int plaquette_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+                const int32_t* muStart, const int32_t* muEnd, const int32_t* nuEnd, double* sumTrP, int32_t* nP,
+                double* time);
+
+/* polyakov, src/wloops.F90:1019-1069; NT < 3 returns FLUE_ENT */
+int polyakov_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+               double* sumTrP, int32_t* nP, double* time);
+
+/* cloverLoopMuNuCoord, src/wloops.F90:129-199: out(NT,NX,NY,NZ,3,3) */
+int cloverloopmunucoord_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                          const flue_c128* data, const int32_t* mu, const int32_t* nu, flue_c128* out);
+
+/* Loop5MuNuCoord, src/wloops.F90:201-432: out(NT,NX,NY,NZ,3,3) */
+int loop5munucoord_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ,
+                     const flue_c128* data, const int32_t* mu, const int32_t* nu, flue_c128* out);
+
+/* magnetic, src/wloops.F90:479-544 */
+int magnetic_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+               double* B);
+
+/* wmunu, src/wloops.F90:679-791 */
+int wmunu_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+            const int32_t* mu, const int32_t* nu, const int32_t* nMu, const int32_t* nNu, double* loopVal);
+
+/* wmunutau, src/wloops.F90:567-677 */
+int wmunutau_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,
+               const int32_t* tau, const int32_t* mu, const int32_t* nu, const int32_t* nMu, const int32_t* nNu,
+               double* loopVal);
+
+/* ------------------------------------------------------------------ */
+/* 2. device entry points (SoA fields)                                  */
+/* ------------------------------------------------------------------ */
+
+size_t flue_field_bytes(const int32_t dims[4], int32_t prec);
+size_t flue_workspace_bytes(const int32_t dims[4]);
+
+/* relayout between a complex array with element strides (in complex elements) for
+ * the index order (t, x, y, z, mu, a, b) and a SoA field; the reference's Fortran
+ * array has strides {1, NT, NT*NX, NT*NX*NY, V, 4V, 12V} (src/wloops.F90:60) */
+int flue_import_field(const int32_t dims[4], const void* src, int32_t src_type, const int64_t strides[7],
+                      void* dst, int32_t prec, void* stream);
+int flue_export_field(const int32_t dims[4], const void* src, int32_t prec, void* dst, int32_t dst_type,
+                      const int64_t strides[7], void* stream);
+int flue_convert_field(const int32_t dims[4], const void* src, int32_t src_prec, void* dst, int32_t dst_prec,
+                       void* stream);
+
+/* seeded synthetic field U = fix(exp(i eps sum_a c_a lambda_a/2)), c_a ~ N(0,1) keyed
+ * by (seed, link) through Philox4x32-10; no reference counterpart (the reference
+ * reads fields from disk, src/gaugeIO/) */
+int flue_random_field(const int32_t dims[4], uint64_t seed, double eps, void* dst, int32_t prec, void* stream);
+
+/* genPlaquette / plaquette (src/wloops.F90:59-127, 915-1017): *d_sum = sum ReTr P */
+int flue_plaquette(const int32_t dims[4], const void* U, int32_t prec, int32_t muStart, int32_t muEnd,
+                   int32_t nuEnd, double* d_sum, void* ws, size_t ws_bytes, void* stream);
+
+/* polyakov (src/wloops.F90:1019-1069): *d_sum = sum_xyz ReTr prod_t U_t */
+int flue_polyakov(const int32_t dims[4], const void* U, int32_t prec, double* d_sum, void* ws, size_t ws_bytes,
+                  void* stream);
+
+/* sum over sites (tau = 0) or over the time slice tau (1-based) of ReTr genericPath
+ * (src/wloops.F90:20-57); with the rectangular path this is wmunu / wmunutau times
+ * the volume (src/wloops.F90:567-791) */
+int flue_path_trace(const int32_t dims[4], const void* U, int32_t prec, const int32_t* path, int32_t npath,
+                    int32_t tau, double* d_sum, void* ws, size_t ws_bytes, void* stream);
+
+/* genericPath at one 1-based base point, transposed, as 9 complex128 in Fortran order
+ * written to device memory d_out */
+int flue_generic_path(const int32_t dims[4], const void* U, int32_t prec, const int32_t coordBase[4],
+                      const int32_t* path, int32_t npath, double* d_out, void* stream);
+
+/* Loop5MuNuCoord (variant 0, src/wloops.F90:201-432) or cloverLoopMuNuCoord
+ * (variant 1, :129-199) for the plane (mu, nu).  layout FLUE_OUT_STRIDED: complex128
+ * at element strides for (t, x, y, z, a, b); FLUE_OUT_SOA: 18 planes of V reals in the
+ * field precision (element a*3+b, re/im) */
+int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, int32_t nu, int32_t variant,
+               void* out, int32_t layout, const int64_t strides[6], void* stream);
+
+/* magnetic (src/wloops.F90:479-544): *d_B = -(1/V) sum_x sum_i ReTr F_i(x)^2 */
+int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_B, void* ws, size_t ws_bytes,
+                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

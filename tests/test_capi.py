@@ -1,0 +1,100 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol the header
+declares, and rejects bad arguments before any device work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "flue_c.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*([a-z_][a-z0-9_]*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "return")))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from flue_lat_amd import _lib
+    return _lib.lib()
+
+
+def test_header_lists_the_hot_path():
+    names = header_functions()
+    for n in ("genplaquette_c", "plaquette_c", "polyakov_c", "genericpath_c", "loop5munucoord_c",
+              "cloverloopmunucoord_c", "magnetic_c", "wmunu_c", "wmunutau_c", "flue_magnetic", "flue_plaquette"):
+        assert n in names
+
+
+def test_library_exports_every_header_symbol(L):
+    for n in header_functions():
+        assert hasattr(L, n), n
+
+
+def test_python_signatures_cover_header():
+    from flue_lat_amd import _lib
+    assert set(header_functions()) <= set(_lib.SIGNATURES)
+
+
+def test_strerror_and_version(L):
+    assert L.flue_strerror(0) == b"ok"
+    assert b"NT >= 3" in L.flue_strerror(4)
+    a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    assert L.flue_version(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) == 0
+    assert (a.value, b.value) == (0, 1)
+
+
+def test_host_helpers_match_reference_semantics():
+    import flue_lat_amd as FL
+    A = np.arange(9).reshape(3, 3) + 1j * np.arange(9, 18).reshape(3, 3)
+    B = np.eye(3) * (2 - 1j) + 0.5
+    np.testing.assert_allclose(FL.gluonCalc.multiplymatmat(A, B), A @ B, atol=1e-13)
+    q = FL.gluonAna.get_qhat([1.0, 2.0, 3.0, 4.0], [8, 8, 8, 16], a=0.5)
+    np.testing.assert_allclose(q, 2 * np.pi / 0.5 * np.array([1, 2, 3, 4]) / np.array([8, 8, 8, 16]))
+    np.testing.assert_array_equal(FL.compiled.flue_c.periodcoord_c([5, 0, 3, 8], [4, 5, 6, 7, 4, 3, 3]),
+                                  [1, 5, 3, 1])
+
+
+def test_propagator_entries_are_out_of_scope():
+    import flue_lat_amd as FL
+    with pytest.raises(NotImplementedError):
+        FL.gluonCalc.scalarGluonProp(np.zeros((2, 2, 2, 2, 4, 3, 3), complex), 1.0)
+    with pytest.raises(NotImplementedError):
+        FL.gluonAna.cone_cut(1, 1, None, None, None, 2, 2, 0.0, 1.0, 0.0, 0.0)
+
+
+def test_argument_errors_before_device_work(L):
+    from flue_lat_amd._lib import FlueError, check
+    import flue_lat_amd as FL
+    U = np.zeros((4, 4, 4, 4, 4, 3, 3), complex)
+    with pytest.raises(FlueError) as e:
+        FL.compiled.flue_c.genericpath_c(U, [1, 1, 1, 1], [1, 5])
+    assert e.value.status == 3
+    with pytest.raises(FlueError) as e:
+        FL.compiled.flue_c.genplaquette_c(U, 0, 4, 4)
+    assert e.value.status == 1
+    d = (ctypes.c_int32 * 4)(4, 4, 0, 4)
+    assert L.flue_field_bytes(d, 0) == 0
+    d = (ctypes.c_int32 * 4)(16, 8, 8, 8)
+    assert L.flue_field_bytes(d, 0) == 72 * 8 * 8192
+    assert L.flue_field_bytes(d, 1) == 72 * 4 * 8192
+    assert L.flue_workspace_bytes(d) > 0
+
+
+def test_no_gpu_is_reported_not_faked(L):
+    import flue_lat_amd as FL
+    from flue_lat_amd._lib import FlueError
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is visible")
+    except ImportError:
+        pass
+    with pytest.raises(FlueError) as e:
+        FL.compiled.flue_c.magnetic_c(np.zeros((3, 3, 3, 3, 4, 3, 3), complex))
+    assert e.value.status == 9

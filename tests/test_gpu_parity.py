@@ -1,0 +1,235 @@
+"""Parity of the HIP path (libflue_hip.so) with the CPU FP64 oracle.
+
+Inputs are the oracle's seeded fields (complex128), imported to the GPU, so both
+sides see identical links.  Tolerances:
+  * FP64 path: 1e-12 relative on lattice sums and per-site F (F scaled by its max);
+  * FP32 links with FP64 sums: 1e-6 relative on the magnetic B and the plaquette sums,
+    2e-5 relative on per-site F and 1e-5 on the Polyakov sum (a product of NT links).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.fields import abelian_flux, f5, unit_field
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def FL():
+    import flue_lat_amd as FL
+    return FL
+
+
+@pytest.fixture(scope="module")
+def c0():
+    # 8^3x16, eps = 0.25
+    return O.random_field((16, 8, 8, 8), 1000, 0.25)
+
+
+@pytest.fixture(scope="module")
+def small():
+    # uneven extents exercise every wrap and stride
+    return O.random_field((6, 5, 7, 4), 1234, 0.35)
+
+
+def gfield(FL, U, prec="fp64"):
+    return FL.GaugeField.from_complex(U, prec)
+
+
+# ------------------------------------------------------------------ field plumbing
+
+def test_import_export_roundtrip_exact(FL, small):
+    g = gfield(FL, small)
+    back = g.to_complex().cpu().numpy()
+    assert np.array_equal(back, small)
+    # a C-ordered torch tensor on the GPU imports identically
+    t = torch.from_numpy(np.ascontiguousarray(small)).cuda()
+    g2 = FL.GaugeField.from_complex(t)
+    assert torch.equal(g2.data, g.data)
+
+
+def test_device_generator_matches_oracle(FL):
+    dims = (6, 5, 4, 3)
+    g = FL.GaugeField.random(dims, 4242, 0.3)
+    U = O.random_field(dims, 4242, 0.3)
+    assert np.abs(g.to_complex().cpu().numpy() - U).max() < 1e-13
+
+
+# ------------------------------------------------------------------ plaquette family
+
+@pytest.mark.parametrize("args", [(1, 4, 4), (2, 4, 4), (1, 1, 4), (1, 2, 4), (3, 3, 4)])
+def test_plaquette_fp64(FL, c0, args):
+    want, n = O.plaquette(c0, *args)
+    got, nP = gfield(FL, c0).plaquette(*args)
+    assert nP == n
+    assert rel(got, want) < 1e-12
+
+
+def test_plaquette_host_dropins(FL, small):
+    fc = FL.compiled.flue_c
+    for args in ((1, 4, 4), (2, 4, 4), (1, 1, 4)):
+        want, n = O.gen_plaquette(small, *args)
+        s, nP, t = fc.genplaquette_c(small, *args)
+        assert nP == n and rel(s, want) < 1e-12 and t >= 0
+        s2, nP2, _ = fc.plaquette_c(small, *args)
+        assert nP2 == n and rel(s2, want) < 1e-12
+
+
+def test_plaquette_unit_and_flux(FL):
+    dims = (4, 6, 5, 4)
+    s, n = gfield(FL, unit_field(dims)).plaquette()
+    assert s == 3.0 * n
+    U, th = abelian_flux(dims, 2, 3, 1)
+    V = np.prod(dims)
+    s, _ = gfield(FL, U).plaquette()
+    assert rel(s, 15 * V + (2 * np.cos(th) + 1) * V) < 1e-13
+
+
+def test_plaquette_fp32(FL, c0):
+    want, _ = O.plaquette(c0, 1, 4, 4)
+    got, _ = gfield(FL, c0, "fp32").plaquette()
+    assert rel(got, want) < 1e-6
+
+
+# ------------------------------------------------------------------ Polyakov
+
+def test_polyakov_fp64(FL, c0, small):
+    for U in (c0, small):
+        want, n = O.polyakov(U)
+        got, nP = gfield(FL, U).polyakov()
+        assert nP == n
+        assert abs(got - want) < 1e-12 * max(1.0, abs(want)) * n
+
+
+def test_polyakov_host_and_short_time(FL, small):
+    want, n = O.polyakov(small)
+    s, nP, _ = FL.compiled.flue_c.polyakov_c(small)
+    assert nP == n and abs(s - want) < 1e-12 * n
+    with pytest.raises(FL.FlueError):
+        FL.compiled.flue_c.polyakov_c(unit_field((2, 3, 3, 3)))
+
+
+def test_polyakov_constant_diagonal(FL):
+    dims = (7, 3, 2, 4)
+    U = unit_field(dims)
+    p1, p2 = 0.3, -1.1
+    U[..., 0, 0, 0] = np.exp(1j * p1)
+    U[..., 0, 1, 1] = np.exp(1j * p2)
+    U[..., 0, 2, 2] = np.exp(-1j * (p1 + p2))
+    s, n = gfield(FL, U).polyakov()
+    assert abs(s - n * (np.cos(7 * p1) + np.cos(7 * p2) + np.cos(7 * (p1 + p2)))) < 1e-12
+
+
+def test_polyakov_fp32(FL, c0):
+    want, n = O.polyakov(c0)
+    got, _ = gfield(FL, c0, "fp32").polyakov()
+    assert abs(got - want) < 1e-5 * n
+
+
+# ------------------------------------------------------------------ generic path / Wilson loops
+
+@pytest.mark.parametrize("path", [[1], [3], [-2], [1, 2, -1, -2], [2, 2, 3, -4, -1, -1, 1, 4],
+                                  [-3, -3, -3, -3, -3, 1, 1, 1, 1], [4, 4, 4, 4, 4, 4, 4, 4]])
+def test_generic_path(FL, small, path):
+    g = gfield(FL, small)
+    for coord in ([1, 1, 1, 1], [6, 5, 7, 4], [2, 3, 1, 4]):
+        want = O.generic_path(small, coord, path)
+        np.testing.assert_allclose(g.generic_path(coord, path), want, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(FL.compiled.flue_c.genericpath_c(small, [2, 3, 1, 4], path),
+                               O.generic_path(small, [2, 3, 1, 4], path), rtol=0, atol=1e-13)
+
+
+def test_generic_path_rejects_bad_direction(FL, small):
+    with pytest.raises(FL.FlueError):
+        FL.compiled.flue_c.genericpath_c(small, [1, 1, 1, 1], [1, 5])
+    with pytest.raises(FL.FlueError):
+        gfield(FL, small).generic_path([1, 1, 1, 1], [0, 1])
+
+
+@pytest.mark.parametrize("mu,nu,R,T", [(2, 1, 1, 1), (3, 1, 2, 3), (4, 2, 3, 1), (1, 4, 6, 6), (2, 3, 5, 2)])
+def test_wilson_loops(FL, small, mu, nu, R, T):
+    g = gfield(FL, small)
+    assert rel(g.wilson_loop(mu, nu, R, T), O.wmunu(small, mu, nu, R, T)) < 1e-12
+    assert rel(g.wilson_loop(mu, nu, R, T, tau=3), O.wmunutau(small, 3, mu, nu, R, T)) < 1e-12
+    fc = FL.compiled.flue_c
+    assert rel(fc.wmunu_c(small, mu, nu, R, T), O.wmunu(small, mu, nu, R, T)) < 1e-12
+    assert rel(fc.wmunutau_c(small, 2, mu, nu, R, T), O.wmunutau(small, 2, mu, nu, R, T)) < 1e-12
+
+
+# ------------------------------------------------------------------ F_munu and B
+
+ALL_PLANES = [(mu, nu) for mu in range(1, 5) for nu in range(1, 5) if mu != nu]
+
+
+@pytest.mark.parametrize("mu,nu", ALL_PLANES)
+def test_loop5_per_site_fp64(FL, small, mu, nu):
+    want = O.loop5_munu(small, mu, nu)
+    got = gfield(FL, small).fmunu(mu, nu).cpu().numpy()
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() < 1e-12 * scale
+
+
+@pytest.mark.parametrize("mu,nu", [(1, 2), (3, 2), (4, 1)])
+def test_clover_per_site_fp64(FL, small, mu, nu):
+    want = O.clover_munu(small, mu, nu)
+    got = gfield(FL, small).fmunu(mu, nu, clover=True).cpu().numpy()
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_loop5_host_dropins(FL, small):
+    fc = FL.compiled.flue_c
+    want = O.loop5_munu(small, 2, 4)
+    assert np.abs(fc.loop5munucoord_c(small, 2, 4) - want).max() < 1e-12 * np.abs(want).max()
+    want = O.clover_munu(small, 3, 1)
+    assert np.abs(fc.cloverloopmunucoord_c(small, 3, 1) - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_loop5_fp32(FL, small):
+    want = O.loop5_munu(small, 2, 3)
+    got = gfield(FL, small, "fp32").fmunu(2, 3).cpu().numpy()
+    assert np.abs(got - want).max() < 2e-5 * np.abs(want).max()
+
+
+def test_magnetic_fp64(FL, c0, small):
+    for U in (c0, small):
+        want = O.magnetic(U)
+        assert rel(gfield(FL, U).magnetic(), want) < 1e-12
+        assert rel(FL.compiled.flue_c.magnetic_c(U), want) < 1e-12
+
+
+def test_magnetic_fp32(FL, c0):
+    want = O.magnetic(c0)
+    assert rel(gfield(FL, c0, "fp32").magnetic(), want) < 1e-6
+
+
+@pytest.mark.parametrize("n", [1, 2, 5])
+def test_magnetic_abelian_flux(FL, n):
+    dims = (2, 7, 6, 3)
+    U, th = abelian_flux(dims, 2, 3, n)
+    assert abs(gfield(FL, U).magnetic() - 2 * f5(th) ** 2) < 1e-13
+    assert gfield(FL, unit_field(dims)).magnetic() == 0.0
+
+
+def test_sums_are_deterministic(FL, c0):
+    g = gfield(FL, c0)
+    a = [g.magnetic() for _ in range(3)]
+    b = [g.plaquette()[0] for _ in range(3)]
+    assert a[0] == a[1] == a[2] and b[0] == b[1] == b[2]
+
+
+def test_wloops_module_accepts_all_field_kinds(FL, small):
+    W = FL.gluonCalc.wloops
+    want = O.magnetic(small)
+    assert rel(W.magnetic(small), want) < 1e-12
+    assert rel(W.magnetic(torch.from_numpy(small).cuda()), want) < 1e-12
+    assert rel(W.magnetic(gfield(FL, small)), want) < 1e-12
+    s, n, _ = W.genPlaquette(torch.from_numpy(small), 2, 4, 4)
+    ws, wn = O.gen_plaquette(small, 2, 4, 4)
+    assert n == wn and rel(s, ws) < 1e-12
