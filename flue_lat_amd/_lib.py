@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libflue_hip.so")
+# FLUE_HIP_LIB may point at another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("FLUE_HIP_LIB") or os.path.join(HERE, "libflue_hip.so")
 
 FLUE_F64, FLUE_F32 = 0, 1
 FLUE_C128, FLUE_C64 = 0, 1
@@ -71,6 +72,7 @@ SIGNATURES = {
     "flue_generic_path": [_P, _P, _I, _P, _P, _I, _P, _P],
     "flue_fmunu": [_P, _P, _I, _I, _I, _I, _P, _I, _P, _P],
     "flue_magnetic": [_P, _P, _I, _P, _P, _SZ, _P],
+    "flue_magnetic_ex": [_P, _P, _I, _I, _P, _P, _SZ, _P],
 }
 _RESTYPE = {"flue_strerror": ctypes.c_char_p, "flue_field_bytes": _SZ, "flue_workspace_bytes": _SZ}
 

@@ -373,6 +373,11 @@ int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, i
 
 int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_B, void* ws, size_t ws_bytes,
                   void* stream) {
+    return flue_magnetic_ex(dims, U, prec, 0, d_B, ws, ws_bytes, stream);
+}
+
+int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B, void* ws,
+                     size_t ws_bytes, void* stream) {
     int rc = check_dims(dims);
     if (rc) return rc;
     if (!U || !d_B) return FLUE_ENULL;
@@ -381,7 +386,8 @@ int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_
     const uint32_t nb = magnetic_blocks(L, prec == FLUE_F32);
     void* w;
     if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
-    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, (double*)w, S(stream)));
+    if (variant < 0 || variant > 3) return FLUE_EINVAL;
+    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, (double*)w, S(stream), variant));
     FLUE_TRY(launch_finalize((double*)w, nb, d_B, 1.0 / (double)L.V, S(stream)));
     return FLUE_OK;
 }

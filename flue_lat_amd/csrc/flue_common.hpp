@@ -51,6 +51,7 @@ struct Mat {
 template <typename R, bool BUF>
 struct FieldView {
     using real = R;
+    static constexpr bool buffered = BUF;
     const R* __restrict__ p;
     uint32_t V;
 
@@ -141,6 +142,35 @@ __device__ __forceinline__ void store_link(R* __restrict__ p, uint32_t V, int mu
 }
 
 // ---------------------------------------------------------------- SU(3) algebra
+//
+// Complex multiply-accumulates are written as explicit FMA chains: an expression such as
+// acc += a*b - c*d would otherwise become a multiply, an FMA and an add.
+
+template <typename R>
+__device__ __forceinline__ R fmad(R a, R b, R c) {
+    if constexpr (sizeof(R) == 4)
+        return __builtin_fmaf(a, b, c);
+    else
+        return __builtin_fma(a, b, c);
+}
+
+// (cr, ci) += (ar + i ai) (br + i bi)
+template <typename R>
+__device__ __forceinline__ void cmac(R& cr, R& ci, R ar, R ai, R br, R bi) {
+    cr = fmad(ar, br, cr);
+    cr = fmad(-ai, bi, cr);
+    ci = fmad(ar, bi, ci);
+    ci = fmad(ai, br, ci);
+}
+
+// (cr, ci) += (ar + i ai) conj(br + i bi)
+template <typename R>
+__device__ __forceinline__ void cmac_c(R& cr, R& ci, R ar, R ai, R br, R bi) {
+    cr = fmad(ar, br, cr);
+    cr = fmad(ai, bi, cr);
+    ci = fmad(ai, br, ci);
+    ci = fmad(-ar, bi, ci);
+}
 
 template <typename R>
 __device__ __forceinline__ Mat<R> mul(const Mat<R>& A, const Mat<R>& B) {
@@ -149,13 +179,12 @@ __device__ __forceinline__ Mat<R> mul(const Mat<R>& A, const Mat<R>& B) {
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            R cr = A.r[a * 3] * B.r[b] - A.i[a * 3] * B.i[b];
-            R ci = A.r[a * 3] * B.i[b] + A.i[a * 3] * B.r[b];
+            R cr = A.r[a * 3] * B.r[b];
+            R ci = A.r[a * 3] * B.i[b];
+            cr = fmad(-A.i[a * 3], B.i[b], cr);
+            ci = fmad(A.i[a * 3], B.r[b], ci);
 #pragma unroll
-            for (int k = 1; k < 3; ++k) {
-                cr += A.r[a * 3 + k] * B.r[k * 3 + b] - A.i[a * 3 + k] * B.i[k * 3 + b];
-                ci += A.r[a * 3 + k] * B.i[k * 3 + b] + A.i[a * 3 + k] * B.r[k * 3 + b];
-            }
+            for (int k = 1; k < 3; ++k) cmac(cr, ci, A.r[a * 3 + k], A.i[a * 3 + k], B.r[k * 3 + b], B.i[k * 3 + b]);
             C.r[a * 3 + b] = cr;
             C.i[a * 3 + b] = ci;
         }
@@ -170,13 +199,13 @@ __device__ __forceinline__ Mat<R> mul_nd(const Mat<R>& A, const Mat<R>& B) {
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            R cr = A.r[a * 3] * B.r[b * 3] + A.i[a * 3] * B.i[b * 3];
-            R ci = A.i[a * 3] * B.r[b * 3] - A.r[a * 3] * B.i[b * 3];
+            R cr = A.r[a * 3] * B.r[b * 3];
+            R ci = A.i[a * 3] * B.r[b * 3];
+            cr = fmad(A.i[a * 3], B.i[b * 3], cr);
+            ci = fmad(-A.r[a * 3], B.i[b * 3], ci);
 #pragma unroll
-            for (int k = 1; k < 3; ++k) {
-                cr += A.r[a * 3 + k] * B.r[b * 3 + k] + A.i[a * 3 + k] * B.i[b * 3 + k];
-                ci += A.i[a * 3 + k] * B.r[b * 3 + k] - A.r[a * 3 + k] * B.i[b * 3 + k];
-            }
+            for (int k = 1; k < 3; ++k)
+                cmac_c(cr, ci, A.r[a * 3 + k], A.i[a * 3 + k], B.r[b * 3 + k], B.i[b * 3 + k]);
             C.r[a * 3 + b] = cr;
             C.i[a * 3 + b] = ci;
         }
@@ -191,13 +220,14 @@ __device__ __forceinline__ Mat<R> mul_dn(const Mat<R>& A, const Mat<R>& B) {
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            R cr = A.r[a] * B.r[b] + A.i[a] * B.i[b];
-            R ci = A.r[a] * B.i[b] - A.i[a] * B.r[b];
+            // sum_k conj(A_ka) B_kb
+            R cr = A.r[a] * B.r[b];
+            R ci = A.r[a] * B.i[b];
+            cr = fmad(A.i[a], B.i[b], cr);
+            ci = fmad(-A.i[a], B.r[b], ci);
 #pragma unroll
-            for (int k = 1; k < 3; ++k) {
-                cr += A.r[k * 3 + a] * B.r[k * 3 + b] + A.i[k * 3 + a] * B.i[k * 3 + b];
-                ci += A.r[k * 3 + a] * B.i[k * 3 + b] - A.i[k * 3 + a] * B.r[k * 3 + b];
-            }
+            for (int k = 1; k < 3; ++k)
+                cmac_c(cr, ci, B.r[k * 3 + b], B.i[k * 3 + b], A.r[k * 3 + a], A.i[k * 3 + a]);
             C.r[a * 3 + b] = cr;
             C.i[a * 3 + b] = ci;
         }
@@ -231,9 +261,13 @@ __device__ __forceinline__ Mat<R> identity() {
 // Re Tr(A B^dagger) = sum_ab Re(A_ab conj(B_ab))
 template <typename R>
 __device__ __forceinline__ R retr_nd(const Mat<R>& A, const Mat<R>& B) {
-    R s = A.r[0] * B.r[0] + A.i[0] * B.i[0];
+    R s = A.r[0] * B.r[0];
+    s = fmad(A.i[0], B.i[0], s);
 #pragma unroll
-    for (int e = 1; e < 9; ++e) s += A.r[e] * B.r[e] + A.i[e] * B.i[e];
+    for (int e = 1; e < 9; ++e) {
+        s = fmad(A.r[e], B.r[e], s);
+        s = fmad(A.i[e], B.i[e], s);
+    }
     return s;
 }
 
@@ -244,7 +278,10 @@ __device__ __forceinline__ R retr_mul(const Mat<R>& A, const Mat<R>& B) {
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) s += A.r[a * 3 + k] * B.r[k * 3 + a] - A.i[a * 3 + k] * B.i[k * 3 + a];
+        for (int k = 0; k < 3; ++k) {
+            s = fmad(A.r[a * 3 + k], B.r[k * 3 + a], s);
+            s = fmad(-A.i[a * 3 + k], B.i[k * 3 + a], s);
+        }
     return s;
 }
 

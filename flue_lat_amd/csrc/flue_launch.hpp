@@ -38,6 +38,8 @@ hipError_t launch_finalize(const double* partials, uint32_t n, double* out, doub
 hipError_t launch_fmunu(const Lat& L, const void* U, int is_f32, int mu, int nu, int clover_only, void* out,
                         int out_layout, const int64_t strides[6], hipStream_t st);
 uint32_t magnetic_blocks(const Lat& L, int is_f32);
-hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st);
+int magnetic_default_variant(int is_f32);
+hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st,
+                           int variant = 0);
 
 }  // namespace flue

@@ -35,30 +35,40 @@ __device__ __forceinline__ void ah_zero(AH<R>& h) {
     for (int k = 0; k < 3; ++k) h.d[k] = h.or_[k] = h.oi[k] = R(0);
 }
 
-// h += w * AH-part of (CA CB^+)
+// h += w * AH-part of (CA CB^+), with W = CA CB^+, W_ab = sum_k CA_ak conj(CB_bk):
+//   d_a += w Im W_aa,  o_ab += w (W_ab - conj(W_ba))
 template <typename R>
 __device__ __forceinline__ void acc_ah(AH<R>& h, R w, const Mat<R>& CA, const Mat<R>& CB) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        R im = R(0);
+        R im = CA.i[a * 3] * CB.r[a * 3];
+        im = fmad(-CA.r[a * 3], CB.i[a * 3], im);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) im += CA.i[a * 3 + k] * CB.r[a * 3 + k] - CA.r[a * 3 + k] * CB.i[a * 3 + k];
-        h.d[a] += w * im;
+        for (int k = 1; k < 3; ++k) {
+            im = fmad(CA.i[a * 3 + k], CB.r[a * 3 + k], im);
+            im = fmad(-CA.r[a * 3 + k], CB.i[a * 3 + k], im);
+        }
+        h.d[a] = fmad(w, im, h.d[a]);
     }
     constexpr int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const int a = pa[q], b = pb[q];
-        R rab = R(0), iab = R(0), rba = R(0), iba = R(0);
+        // re(W_ab) - re(W_ba) and im(W_ab) + im(W_ba) accumulated in one chain each
+        R xr = R(0), xi = R(0);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            rab += CA.r[a * 3 + k] * CB.r[b * 3 + k] + CA.i[a * 3 + k] * CB.i[b * 3 + k];
-            iab += CA.i[a * 3 + k] * CB.r[b * 3 + k] - CA.r[a * 3 + k] * CB.i[b * 3 + k];
-            rba += CA.r[b * 3 + k] * CB.r[a * 3 + k] + CA.i[b * 3 + k] * CB.i[a * 3 + k];
-            iba += CA.i[b * 3 + k] * CB.r[a * 3 + k] - CA.r[b * 3 + k] * CB.i[a * 3 + k];
+            xr = fmad(CA.r[a * 3 + k], CB.r[b * 3 + k], xr);
+            xr = fmad(CA.i[a * 3 + k], CB.i[b * 3 + k], xr);
+            xr = fmad(-CA.r[b * 3 + k], CB.r[a * 3 + k], xr);
+            xr = fmad(-CA.i[b * 3 + k], CB.i[a * 3 + k], xr);
+            xi = fmad(CA.i[a * 3 + k], CB.r[b * 3 + k], xi);
+            xi = fmad(-CA.r[a * 3 + k], CB.i[b * 3 + k], xi);
+            xi = fmad(CA.i[b * 3 + k], CB.r[a * 3 + k], xi);
+            xi = fmad(-CA.r[b * 3 + k], CB.i[a * 3 + k], xi);
         }
-        h.or_[q] += w * (rab - rba);
-        h.oi[q] += w * (iab + iba);
+        h.or_[q] = fmad(w, xr, h.or_[q]);
+        h.oi[q] = fmad(w, xi, h.oi[q]);
     }
 }
 
@@ -144,7 +154,16 @@ __device__ __forceinline__ Mat<typename FV::real> ext(const Quad<FV>& q, const M
         return mul_nd(M, U);
 }
 
+// Scheduling fences keep each group's link loads next to the products that use them;
+// without them the scheduler front-loads a quadrant's loads and runs out of registers.
+#ifndef FLUE_FENCE_MODE
+#define FLUE_FENCE_MODE 1
+#endif
+#if FLUE_FENCE_MODE == 1
 #define FLUE_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define FLUE_SCHED_FENCE() ((void)0)
+#endif
 
 // the seven rectangles (1,1),(1,2),(1,3),(2,1),(2,2),(3,1),(3,3), a along d1 and b along
 // d2, or only the 1x1 leaf when CLOVER is set (cloverLoopMuNuCoord, src/wloops.F90:129-199)
@@ -223,14 +242,11 @@ __device__ __forceinline__ double f_norm2(const AH<R>& h) {
 }
 
 constexpr int kBS = 256;
-#ifndef FLUE_WPE
-#define FLUE_WPE 2
-#endif
 
 // B = -(1/V) sum_x sum_i ReTr(F_i F_i) over the planes (y,x), (x,z), (z,y)
 // (src/wloops.F90:502-506, 530-543); this kernel writes one partial sum per block.
-template <typename FV>
-__global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(FLUE_WPE, FLUE_WPE)))
+template <typename FV, int WPE>
+__global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 magnetic_kernel(Lat L, FV f, double* __restrict__ partials) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kBS + threadIdx.x;
@@ -262,7 +278,7 @@ struct OutStrides {
 // per-site F for one plane.  layout 0: complex128 at the given strides; layout 1: SoA
 // planes (element e = a*3+b, re/im) in the field precision.
 template <typename FV, bool CLOVER>
-__global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(FLUE_WPE, FLUE_WPE)))
+__global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(sizeof(typename FV::real) == 4 ? 2 : 1)))
 fmunu_kernel(Lat L, FV f, int mu, int nu, void* __restrict__ out, int layout, OutStrides os) {
     using R = typename FV::real;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -309,9 +325,28 @@ fmunu_kernel(Lat L, FV f, int mu, int nu, void* __restrict__ out, int layout, Ou
 
 uint32_t magnetic_blocks(const Lat& L, int) { return (L.V + kBS - 1) / kBS; }
 
-hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st) {
+int magnetic_default_variant(int is_f32) { return is_f32 ? 2 : 1; }
+
+// variant = occupancy target in waves per SIMD (1, 2 or 3); 0 picks the default
+hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st, int variant) {
     const uint32_t g = magnetic_blocks(L, is_f32);
-    with_view(L.V, U, is_f32, [&](auto f) { magnetic_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, partials); });
+    if (variant == 0) variant = magnetic_default_variant(is_f32);
+    if (variant < 1 || variant > 3) return hipErrorInvalidValue;
+    with_view(L.V, U, is_f32, [&](auto f) {
+        using F = decltype(f);
+        using R = typename F::real;
+        if constexpr (!F::buffered) {
+            // fields beyond the buffer-resource range: one variant only
+            magnetic_kernel<F, sizeof(R) == 4 ? 2 : 1><<<g, kBS, 0, st>>>(L, f, partials);
+        } else {
+            if (variant == 1)
+                magnetic_kernel<F, 1><<<g, kBS, 0, st>>>(L, f, partials);
+            else if (variant == 2)
+                magnetic_kernel<F, 2><<<g, kBS, 0, st>>>(L, f, partials);
+            else
+                magnetic_kernel<F, 3><<<g, kBS, 0, st>>>(L, f, partials);
+        }
+    });
     return hipGetLastError();
 }
 

@@ -188,11 +188,12 @@ class GaugeField:
                                         _ptr(out), FLUE_OUT_STRIDED, _i64(out.stride()), _stream()), "flue_fmunu")
         return out
 
-    def magnetic(self, sync=True):
-        """B = -(1/V) sum_x sum_i ReTr F_i^2 over the spatial planes (src/wloops.F90:479-544)."""
+    def magnetic(self, sync=True, variant=0):
+        """B = -(1/V) sum_x sum_i ReTr F_i^2 over the spatial planes (src/wloops.F90:479-544).
+        variant selects the kernel's occupancy target (0 = library choice)."""
         out = self._scalar()
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_magnetic(self._cdims, _ptr(self.data), self.prec, _ptr(out), _ptr(ws), ws.numel(),
-                                           _stream()), "flue_magnetic")
+            check(_lib.lib().flue_magnetic_ex(self._cdims, _ptr(self.data), self.prec, int(variant), _ptr(out),
+                                              _ptr(ws), ws.numel(), _stream()), "flue_magnetic_ex")
         return out.item() if sync else out

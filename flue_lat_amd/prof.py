@@ -1,0 +1,55 @@
+"""Profiling driver: runs the magnetic (5-loop F_munu + B^2) measurement a few times
+on a synthetic field and, for calibrating memory counters, one plain device copy of
+known size.  Meant to sit under a profiler:
+
+    rocprofv3 --kernel-trace --stats -d out -- python3 -m flue_lat_amd.prof
+    rocprofv3 --pmc FETCH_SIZE -d out -- python3 -m flue_lat_amd.prof --prec fp32
+"""
+import argparse
+import sys
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dims", type=int, nargs=4, default=[96, 48, 48, 48], help="NT NX NY NZ")
+    ap.add_argument("--prec", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=1024, help="size of the calibration copy (0 = none)")
+    ap.add_argument("--op", choices=["magnetic", "plaquette", "polyakov", "fmunu"], default="magnetic")
+    a = ap.parse_args(argv)
+
+    import torch
+
+    from .field import GaugeField
+
+    torch.cuda.set_device(0)
+    f64 = GaugeField.random(a.dims, 1003, 0.3, "fp64")
+    fields = []
+    if a.prec in ("fp32", "both"):
+        fields.append(f64.astype("fp32"))
+    if a.prec in ("fp64", "both"):
+        fields.append(f64)
+    for g in fields:
+        for _ in range(a.reps):
+            if a.op == "magnetic":
+                v = g.magnetic()
+            elif a.op == "plaquette":
+                v = g.plaquette()[0]
+            elif a.op == "polyakov":
+                v = g.polyakov()[0]
+            else:
+                v = float(g.fmunu(2, 3).abs().sum())
+        print(f"{a.op} {g.precision} {tuple(a.dims)}: {v!r}", flush=True)
+    if a.copy_mib > 0:
+        n = a.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy {a.copy_mib} MiB done", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

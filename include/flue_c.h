@@ -186,6 +186,11 @@ int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, i
 int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_B, void* ws, size_t ws_bytes,
                   void* stream);
 
+/* flue_magnetic with an explicit kernel variant: the occupancy target in waves per
+ * SIMD (1, 2 or 3; 0 = the library's choice).  Results do not depend on it. */
+int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B, void* ws,
+                     size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
