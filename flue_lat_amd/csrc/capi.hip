@@ -392,6 +392,137 @@ int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t
     return FLUE_OK;
 }
 
+// ---------------------------------------------------------------- gradients and flow
+
+size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec) {
+    if (check_dims(dims) || check_prec(prec)) return 0;
+    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
+    return (size_t)(27 * V * (prec == FLUE_F32 ? 4 : 8));
+}
+
+size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec) {
+    if (check_dims(dims) || check_prec(prec)) return 0;
+    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
+    // a second link buffer and the FP64 RK3 accumulator (8 su(3) coordinates per link)
+    return (size_t)(V * kPlanes * (prec == FLUE_F32 ? 4 : 8) + V * 32 * 8);
+}
+
+int flue_wilson_action(const int32_t dims[4], const void* U, int32_t prec, double beta, double* d_S, void* ws,
+                       size_t ws_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !d_S) return FLUE_ENULL;
+    if (check_prec(prec) || !std::isfinite(beta)) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    const uint32_t nb = plaquette_blocks(L);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
+    FLUE_TRY(launch_plaquette(L, U, prec == FLUE_F32, 0, 3, 3, (double*)w, S(stream)));
+    // S = beta * sum_p (1 - ReTr P / 3) over the 6V plaquettes
+    FLUE_TRY(launch_finalize_affine((double*)w, nb, d_S, -beta / 3.0, beta * 6.0 * (double)L.V, S(stream)));
+    return FLUE_OK;
+}
+
+int flue_wilson_grad(const int32_t dims[4], const void* U, int32_t prec, double beta, void* grad, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !grad) return FLUE_ENULL;
+    if (check_prec(prec) || !std::isfinite(beta)) return FLUE_EINVAL;
+    return hip_status(launch_wilson_grad(make_lat(dims), U, prec == FLUE_F32, beta, grad, S(stream)));
+}
+
+int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad, void* work, size_t work_bytes,
+                 void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !grad) return FLUE_ENULL;
+    if (check_prec(prec)) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    void* A;
+    if ((rc = get_ws(work, work_bytes, flue_b2_work_bytes(dims, prec), &A))) return rc;
+    FLUE_TRY(launch_magnetic_ah(L, U, prec == FLUE_F32, A, S(stream)));
+    FLUE_TRY(launch_b2_grad(L, U, prec == FLUE_F32, A, grad, S(stream)));
+    return FLUE_OK;
+}
+
+int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* Wout, double* acc, double eps,
+                    int32_t stage, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!Win || !Wout || !acc) return FLUE_ENULL;
+    if (Win == Wout) return FLUE_EINVAL;
+    if (check_prec(prec) || stage < 0 || stage > 2 || !std::isfinite(eps)) return FLUE_EINVAL;
+    // Luescher RK3: acc <- ca acc + cb Z, (ca, cb) = (0, 1/4), (-17/9, 8/9), (-1, 3/4)
+    static const double ca[3] = {0.0, -17.0 / 9.0, -1.0};
+    static const double cb[3] = {0.25, 8.0 / 9.0, 0.75};
+    return hip_status(launch_flow_stage(make_lat(dims), Win, prec == FLUE_F32, Wout, acc, eps, ca[stage], cb[stage],
+                                        stage == 2, S(stream)));
+}
+
+int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, int32_t nsteps, int32_t use_graph,
+                     void* work, size_t work_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U) return FLUE_ENULL;
+    if (check_prec(prec) || nsteps < 0 || !std::isfinite(eps)) return FLUE_EINVAL;
+    void* w;
+    if ((rc = get_ws(work, work_bytes, flue_flow_work_bytes(dims, prec), &w))) return rc;
+    const Lat L = make_lat(dims);
+    void* W2 = w;
+    double* acc = (double*)((char*)w + (size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8));
+    hipStream_t st = S(stream);
+    // three stages per step alternate between U and W2, so two steps return the field to U
+    auto two_steps = [&]() -> int {
+        void* bufs[2] = {U, W2};
+        for (int k = 0; k < 6; ++k) {
+            int r = flue_flow_stage(dims, bufs[k & 1], prec, bufs[(k + 1) & 1], acc, eps, k % 3, st);
+            if (r) return r;
+        }
+        return FLUE_OK;
+    };
+    int32_t pairs = nsteps / 2;
+    if (use_graph && pairs > 0) {
+        if (st == nullptr) return FLUE_EINVAL;  // capture needs a non-default stream
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        FLUE_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int r = two_steps();
+        hipGraph_t g2 = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g2);
+        if (r) {
+            if (g2) (void)hipGraphDestroy(g2);
+            return r;
+        }
+        FLUE_TRY(e);
+        graph = g2;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return hip_status(e);
+        }
+        for (int32_t i = 0; i < pairs && e == hipSuccess; ++i) e = hipGraphLaunch(exec, st);
+        hipError_t e2 = hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        FLUE_TRY(e);
+        FLUE_TRY(e2);
+    } else {
+        for (int32_t i = 0; i < pairs; ++i)
+            if ((rc = two_steps())) return rc;
+    }
+    if (nsteps % 2) {
+        // last odd step: run it into W2 and copy back
+        for (int k = 0; k < 3; ++k) {
+            void* in = (k % 2 == 0) ? U : W2;
+            void* out = (k % 2 == 0) ? W2 : U;
+            if ((rc = flue_flow_stage(dims, in, prec, out, acc, eps, k, st))) return rc;
+        }
+        FLUE_TRY(hipMemcpyAsync(U, W2, (size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8), hipMemcpyDeviceToDevice,
+                                st));
+    }
+    return FLUE_OK;
+}
+
 // ---------------------------------------------------------------- host drop-ins
 
 int genericpath_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const int32_t* NZ, const flue_c128* data,

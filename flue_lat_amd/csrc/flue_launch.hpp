@@ -42,4 +42,15 @@ int magnetic_default_variant(int is_f32);
 hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st,
                            int variant = 0);
 
+// gradients and flow (k_grad.hip, k_b2grad.hip); grad / Wout / Abuf are SoA in the field
+// precision, acc is FP64 (8 su(3) coordinates x 4 directions x V)
+uint32_t grad_blocks(const Lat& L);
+hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st);
+hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, double* acc, double eps,
+                             double ca, double cb, int reunit, hipStream_t st);
+hipError_t launch_magnetic_ah(const Lat& L, const void* U, int is_f32, void* Abuf, hipStream_t st);
+hipError_t launch_b2_grad(const Lat& L, const void* U, int is_f32, const void* Abuf, void* grad, hipStream_t st);
+hipError_t launch_finalize_affine(const double* partials, uint32_t n, double* out, double scale, double offset,
+                                  hipStream_t st);
+
 }  // namespace flue

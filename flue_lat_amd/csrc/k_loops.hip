@@ -156,13 +156,13 @@ __global__ void generic_path_kernel(Lat L, FV f, PathArg p, int t, int x, int y,
 
 // fixed-order compensated sum of per-block partials
 __global__ void __launch_bounds__(kBS) finalize_kernel(const double* __restrict__ partials, uint32_t n,
-                                                       double* __restrict__ out, double scale) {
+                                                       double* __restrict__ out, double scale, double offset) {
     const uint32_t lo = (uint32_t)(((uint64_t)n * threadIdx.x) / kBS);
     const uint32_t hi = (uint32_t)(((uint64_t)n * (threadIdx.x + 1)) / kBS);
     DD acc = {0.0, 0.0};
     for (uint32_t i = lo; i < hi; ++i) acc = dd_add(acc, DD{partials[2 * i], partials[2 * i + 1]});
     DD r = block_sum<kBS>(acc);
-    if (threadIdx.x == 0) out[0] = (r.hi + r.lo) * scale;
+    if (threadIdx.x == 0) out[0] = offset + (r.hi + r.lo) * scale;
 }
 
 PathArg make_path(const int32_t* path, int npath) {
@@ -216,7 +216,13 @@ hipError_t launch_generic_path(const Lat& L, const void* U, int is_f32, const in
 }
 
 hipError_t launch_finalize(const double* partials, uint32_t n, double* out, double scale, hipStream_t st) {
-    finalize_kernel<<<1, kBS, 0, st>>>(partials, n, out, scale);
+    finalize_kernel<<<1, kBS, 0, st>>>(partials, n, out, scale, 0.0);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_affine(const double* partials, uint32_t n, double* out, double scale, double offset,
+                                  hipStream_t st) {
+    finalize_kernel<<<1, kBS, 0, st>>>(partials, n, out, scale, offset);
     return hipGetLastError();
 }
 

@@ -197,3 +197,59 @@ class GaugeField:
             check(_lib.lib().flue_magnetic_ex(self._cdims, _ptr(self.data), self.prec, int(variant), _ptr(out),
                                               _ptr(ws), ws.numel(), _stream()), "flue_magnetic_ex")
         return out.item() if sync else out
+
+    # ------------------------------------------------------------ gradients and flow
+    def _like(self):
+        return GaugeField(self.dims, self.precision, self.device)
+
+    def wilson_action(self, beta, sync=True):
+        """S = beta * sum_p (1 - ReTr P / 3) over all 6V plaquettes."""
+        out = self._scalar()
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_wilson_action(self._cdims, _ptr(self.data), self.prec, float(beta), _ptr(out),
+                                                _ptr(ws), ws.numel(), _stream()), "flue_wilson_action")
+        return out.item() if sync else out
+
+    def wilson_grad(self, beta):
+        """Lie gradient of the Wilson action, G = (beta/6) P_TA(U Sigma), as a GaugeField-shaped
+        su(3) field (same SoA layout and precision)."""
+        g = self._like()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_wilson_grad(self._cdims, _ptr(self.data), self.prec, float(beta), _ptr(g.data),
+                                              _stream()), "flue_wilson_grad")
+        return g
+
+    def magnetic_grad(self):
+        """Lie gradient of the magnetic B (temporal links get 0)."""
+        g = self._like()
+        n = _lib.lib().flue_b2_work_bytes(self._cdims, self.prec)
+        work = torch.empty(n, dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_b2_grad(self._cdims, _ptr(self.data), self.prec, _ptr(g.data), _ptr(work), n,
+                                          _stream()), "flue_b2_grad")
+        return g
+
+    def wilson_flow(self, eps, nsteps, use_graph=True):
+        """nsteps Luescher RK3 Wilson-flow steps of size eps, in place.  With use_graph the
+        steps are replayed from a captured hipGraph."""
+        n = _lib.lib().flue_flow_work_bytes(self._cdims, self.prec)
+        work = torch.empty(n, dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream()
+            if use_graph and stream.cuda_stream == 0:
+                side = torch.cuda.Stream(device=self.device)
+                side.wait_stream(stream)
+                with torch.cuda.stream(side):
+                    check(_lib.lib().flue_wilson_flow(self._cdims, _ptr(self.data), self.prec, float(eps), int(nsteps),
+                                                      1, _ptr(work), n, _stream()), "flue_wilson_flow")
+                stream.wait_stream(side)
+            else:
+                check(_lib.lib().flue_wilson_flow(self._cdims, _ptr(self.data), self.prec, float(eps), int(nsteps),
+                                                  1 if use_graph else 0, _ptr(work), n, _stream()), "flue_wilson_flow")
+        return self
+
+    def clone(self):
+        g = self._like()
+        g.data.copy_(self.data)
+        return g

@@ -191,6 +191,44 @@ int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_
 int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B, void* ws,
                      size_t ws_bytes, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* 3. gradients and Wilson flow (no counterpart in the reference)       */
+/* ------------------------------------------------------------------ */
+/* Gradient convention: for a link U = U_mu(x), D^a f = d/ds f(exp(s T^a) U) at s = 0
+ * with T^a = -i lambda_a / 2; the gradient field holds G = sum_a T^a D^a f per link
+ * (traceless anti-Hermitian, SoA layout and precision of the input field), so
+ * D^a f = -2 Tr(T^a G).  The Wilson action is S = beta sum_p (1 - ReTr P / 3), the
+ * plaquette sum of genPlaquette (src/wloops.F90:59-127) rescaled. */
+
+size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec);
+size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec);
+
+/* *d_S = beta * sum_x sum_{mu<nu} (1 - ReTr P_munu(x) / 3) */
+int flue_wilson_action(const int32_t dims[4], const void* U, int32_t prec, double beta, double* d_S, void* ws,
+                       size_t ws_bytes, void* stream);
+
+/* G = (beta/6) P_TA(U Sigma) per link, Sigma the staple sum */
+int flue_wilson_grad(const int32_t dims[4], const void* U, int32_t prec, double beta, void* grad, void* stream);
+
+/* gradient of the magnetic B (flue_magnetic); temporal links get 0.  `work` holds the
+ * per-site anti-Hermitian parts of the three spatial-plane leaf sums
+ * (flue_b2_work_bytes; NULL = library buffer) */
+int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad, void* work, size_t work_bytes,
+                 void* stream);
+
+/* one stage (0, 1, 2) of Luescher's RK3 Wilson flow with step eps, reading Win and
+ * writing Wout (distinct buffers); acc is the FP64 accumulator (32 * V doubles) the
+ * three stages of a step share.  The exponential map, the product and the
+ * reunitarisation after stage 2 are done in FP64. */
+int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* Wout, double* acc, double eps,
+                    int32_t stage, void* stream);
+
+/* nsteps RK3 steps in place on U.  use_graph != 0 captures two steps in a hipGraph and
+ * replays it (needs a non-default stream); `work` is flue_flow_work_bytes (NULL = library
+ * buffer, which must then exist before any capture). */
+int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, int32_t nsteps, int32_t use_graph,
+                     void* work, size_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

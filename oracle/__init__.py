@@ -273,8 +273,10 @@ def b2_grad(U):
 
 
 def fd_link(U, which, coord, mu, a, h=1e-6, beta=6.0):
-    """Central difference of the Wilson action (which=0) or B (which=1) along exp(+-h T^a) on U_mu(coord)."""
-    U = _field(U)
+    """Central difference of the Wilson action (which=0) or B (which=1) along exp(+-h T^a) on U_mu(coord).
+    A Fortran-ordered complex128 U is perturbed in place and restored; anything else is copied."""
+    if not (isinstance(U, np.ndarray) and U.dtype == np.complex128 and U.flags.f_contiguous and U.flags.writeable):
+        U = _field(U).copy(order="F")
     c = np.asarray(coord, np.int32)
     d = np.zeros(1, np.float64)
     _check(lib().or_fd_link(_ptr(U), _ptr(_dims(U)), which, beta, _ptr(c), mu, a, h, _ptr(d)), "fd_link")

@@ -880,11 +880,12 @@ static double local_wilson(const cplx* data, const int32_t d[4], const int32_t c
     return -beta / 3.0 * or_retrace_mult(U, St);
 }
 
-static double local_b2(const cplx* data, const int32_t d[4], const int32_t c[4]) {
-    /* sum of -ReTr F^2 over the distinct sites within 3 of c in each spatial direction, same t */
+/* -ReTr F^2 (three magnetic planes) at each distinct site within 3 of c in each spatial
+ * direction at the same t, written to vals in a fixed order; returns the count */
+static int local_b2_values(const cplx* data, const int32_t d[4], const int32_t c[4], double* vals) {
     int64_t Vs = (int64_t)d[1] * d[2] * d[3];
     unsigned char* seen = (unsigned char*)calloc((size_t)Vs, 1);
-    double s = 0.0;
+    int n = 0;
     for (int dz = -3; dz <= 3; ++dz)
         for (int dy = -3; dy <= 3; ++dy)
             for (int dx = -3; dx <= 3; ++dx) {
@@ -896,35 +897,45 @@ static double local_b2(const cplx* data, const int32_t d[4], const int32_t c[4])
                 int64_t sp = (e[1] - 1) + (int64_t)d[1] * ((e[2] - 1) + (int64_t)d[2] * (e[3] - 1));
                 if (seen[sp]) continue;
                 seen[sp] = 1;
-                s += magnetic_site(data, d, e);
+                vals[n++] = magnetic_site(data, d, e);
             }
     free(seen);
-    return s;
+    return n;
 }
 
 /* central finite difference of observable `which` (0: Wilson action with beta, 1: B) along
- * exp(+-h T^a) U_mu(coord), coord 1-based, mu 1-based, a = 1..8 */
-int or_fd_link(const cplx* data, const int32_t dims[4], int32_t which, double beta, const int32_t coord[4],
+ * exp(+-h T^a) U_mu(coord), coord 1-based, mu 1-based, a = 1..8.  The link is perturbed in
+ * place and restored, so `data` must be writable. */
+int or_fd_link(cplx* data, const int32_t dims[4], int32_t which, double beta, const int32_t coord[4],
                int32_t mu, int32_t a, double h, double* deriv) {
     if (check_dims(dims) || mu < 1 || mu > 4 || a < 1 || a > 8) return ORC_EINVAL;
     int64_t V = vol4(dims);
-    cplx* work = (cplx*)malloc(sizeof(cplx) * 36 * (size_t)V);
-    if (!work) return ORC_EINVAL;
-    memcpy(work, data, sizeof(cplx) * 36 * (size_t)V);
     cplx U0[9], lam[9], X[9], E[9], U1[9];
     load_link(data, dims, coord, mu, U0);
     or_gellmann(a, lam);
-    double val[2];
+    /* B is differenced site by site before summing, so the result is not limited by the
+     * rounding of the whole local sum */
+    double val[2], b2v[2][343];
+    int nb = 0;
     for (int sgn = 0; sgn < 2; ++sgn) {
         double hh = sgn == 0 ? h : -h;
         for (int i = 0; i < 9; ++i) X[i] = -I * 0.5 * hh * lam[i]; /* h T^a */
         or_su3_exp(X, E);
         or_matmul(U1, E, U0);
-        store_link(work, dims, coord, mu, U1);
-        val[sgn] = which == 0 ? local_wilson(work, dims, coord, mu, beta) : local_b2(work, dims, coord) / (double)V;
+        store_link(data, dims, coord, mu, U1);
+        if (which == 0)
+            val[sgn] = local_wilson(data, dims, coord, mu, beta);
+        else
+            nb = local_b2_values(data, dims, coord, b2v[sgn]);
     }
-    free(work);
-    *deriv = (val[0] - val[1]) / (2.0 * h);
+    store_link(data, dims, coord, mu, U0);
+    if (which == 0) {
+        *deriv = (val[0] - val[1]) / (2.0 * h);
+    } else {
+        double s = 0.0;
+        for (int i = 0; i < nb; ++i) s += b2v[0][i] - b2v[1][i];
+        *deriv = s / (double)V / (2.0 * h);
+    }
     return ORC_OK;
 }
 
@@ -953,7 +964,7 @@ int or_wilson_flow(cplx* data, const int32_t dims[4], double eps, int32_t nsteps
     if (check_dims(dims) || nsteps < 0) return ORC_EINVAL;
     int64_t V = vol4(dims);
     cplx* Z = (cplx*)malloc(sizeof(cplx) * 36 * (size_t)V);
-    cplx* acc = (cplx*)malloc(sizeof(cplx) * 36 * (size_t)V);
+    cplx* acc = (cplx*)calloc((size_t)36 * (size_t)V, sizeof(cplx));
     if (!Z || !acc) {
         free(Z);
         free(acc);
@@ -973,7 +984,8 @@ int or_wilson_flow(cplx* data, const int32_t dims[4], double eps, int32_t nsteps
                 for (int mu = 1; mu <= 4; ++mu) {
                     cplx* A = acc + 36 * s + 9 * (mu - 1);
                     const cplx* z = Z + 36 * s + 9 * (mu - 1);
-                    for (int i = 0; i < 9; ++i) A[i] = ca[st] * A[i] + cb[st] * eps * z[i];
+                    /* stage 0 starts a fresh accumulator (never scale what is in it) */
+                    for (int i = 0; i < 9; ++i) A[i] = (st == 0 ? 0.0 : ca[st] * A[i]) + cb[st] * eps * z[i];
                     cplx E[9], U[9], W[9];
                     or_su3_exp(A, E);
                     load_link(data, dims, c, mu, U);

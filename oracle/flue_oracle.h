@@ -68,7 +68,7 @@ void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 int or_wilson_action(const or_cplx* data, const int32_t dims[4], double beta, double* S);
 int or_wilson_grad(const or_cplx* data, const int32_t dims[4], double beta, or_cplx* grad);
 int or_b2_grad(const or_cplx* data, const int32_t dims[4], or_cplx* grad);
-int or_fd_link(const or_cplx* data, const int32_t dims[4], int32_t which, double beta,
+int or_fd_link(or_cplx* data, const int32_t dims[4], int32_t which, double beta,
                const int32_t coord[4], int32_t mu, int32_t a, double h, double* deriv);
 
 /* Wilson flow, Luescher RK3 with Z = -P_TA(U * staple-sum) */
