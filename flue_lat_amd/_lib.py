@@ -80,9 +80,14 @@ SIGNATURES = {
     "flue_b2_grad": [_P, _P, _I, _P, _P, _SZ, _P],
     "flue_flow_stage": [_P, _P, _I, _P, _P, _D, _I, _P],
     "flue_wilson_flow": [_P, _P, _I, _D, _I, _I, _P, _SZ, _P],
+    "flue_ensemble_field_bytes": [_I, _P, _I],
+    "flue_ensemble_work_bytes": [_I, _P, _I],
+    "flue_ensemble_random": [_I, _P, _P, _D, _P, _I, _P],
+    "flue_ensemble_flow": [_I, _P, _P, _I, _D, _I, _I, _P, _P, _I, _P, _SZ, _P],
 }
 _RESTYPE = {"flue_strerror": ctypes.c_char_p, "flue_field_bytes": _SZ, "flue_workspace_bytes": _SZ,
-            "flue_b2_work_bytes": _SZ, "flue_flow_work_bytes": _SZ}
+            "flue_b2_work_bytes": _SZ, "flue_flow_work_bytes": _SZ, "flue_ensemble_field_bytes": _SZ,
+            "flue_ensemble_work_bytes": _SZ}
 
 
 def lib():

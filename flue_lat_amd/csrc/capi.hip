@@ -1,8 +1,10 @@
 // capi.hip -- the extern "C" surface of libflue_hip.so (declared in include/flue_c.h).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "../../include/flue_c.h"
 #include "flue_launch.hpp"
@@ -519,6 +521,155 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
         }
         FLUE_TRY(hipMemcpyAsync(U, W2, (size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8), hipMemcpyDeviceToDevice,
                                 st));
+    }
+    return FLUE_OK;
+}
+
+// ---------------------------------------------------------------- ensembles
+
+namespace {
+
+struct EnsPlan {
+    std::vector<EnsEntry> tab;
+    uint64_t field_reals = 0, acc_doubles = 0;
+    uint32_t total_blocks = 0, maxV = 0;
+};
+
+int ens_plan(int32_t n, const int32_t* dims, EnsPlan& p) {
+    if (n < 1 || n > 65536) return FLUE_EINVAL;
+    if (!dims) return FLUE_ENULL;
+    p.tab.resize(n);
+    uint64_t blocks = 0;
+    for (int c = 0; c < n; ++c) {
+        int rc = check_dims(dims + 4 * c);
+        if (rc) return rc;
+        EnsEntry& e = p.tab[c];
+        e.L = make_lat(dims + 4 * c);
+        e.off = p.field_reals;
+        e.acc_off = p.acc_doubles;
+        e.blk0 = (uint32_t)blocks;
+        e.nblk = (e.L.V + 255) / 256;
+        p.field_reals += (uint64_t)e.L.V * kPlanes;
+        p.acc_doubles += (uint64_t)e.L.V * 32;
+        blocks += e.nblk;
+        p.maxV = std::max(p.maxV, e.L.V);
+    }
+    if (blocks > 0xFFFFFFFFull) return FLUE_EDIM;
+    p.total_blocks = (uint32_t)blocks;
+    return FLUE_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+size_t flue_ensemble_field_bytes(int32_t n, const int32_t* dims, int32_t prec) {
+    EnsPlan p;
+    if (ens_plan(n, dims, p) || check_prec(prec)) return 0;
+    return (size_t)p.field_reals * (prec == FLUE_F32 ? 4 : 8);
+}
+
+size_t flue_ensemble_work_bytes(int32_t n, const int32_t* dims, int32_t prec) {
+    EnsPlan p;
+    if (ens_plan(n, dims, p) || check_prec(prec)) return 0;
+    // table, second link buffer, FP64 accumulators, per-block partial sums
+    return align256(sizeof(EnsEntry) * n) + align256((size_t)p.field_reals * (prec == FLUE_F32 ? 4 : 8)) +
+           align256((size_t)p.acc_doubles * 8) + align256((size_t)p.total_blocks * 16);
+}
+
+int flue_ensemble_random(int32_t n, const int32_t* dims, const uint64_t* seeds, double eps, void* fields, int32_t prec,
+                         void* stream) {
+    EnsPlan p;
+    int rc = ens_plan(n, dims, p);
+    if (rc) return rc;
+    if (!seeds || !fields) return FLUE_ENULL;
+    if (check_prec(prec) || !std::isfinite(eps)) return FLUE_EINVAL;
+    const size_t sz = prec == FLUE_F32 ? 4 : 8;
+    for (int c = 0; c < n; ++c)
+        FLUE_TRY(launch_random_field(p.tab[c].L, seeds[c], eps, (char*)fields + p.tab[c].off * sz, prec == FLUE_F32,
+                                     S(stream)));
+    return FLUE_OK;
+}
+
+int flue_ensemble_flow(int32_t n, const int32_t* dims, void* fields, int32_t prec, double eps, int32_t nsteps,
+                       int32_t measure_every, double* d_plaq, double* d_B, int32_t use_graph, void* work,
+                       size_t work_bytes, void* stream) {
+    EnsPlan p;
+    int rc = ens_plan(n, dims, p);
+    if (rc) return rc;
+    if (!fields) return FLUE_ENULL;
+    if (check_prec(prec) || nsteps < 0 || measure_every < 0 || !std::isfinite(eps)) return FLUE_EINVAL;
+    if (measure_every > 0 && (!d_plaq || !d_B)) return FLUE_ENULL;
+    const int is_f32 = prec == FLUE_F32;
+    const size_t sz = is_f32 ? 4 : 8;
+    void* w;
+    if ((rc = get_ws(work, work_bytes, flue_ensemble_work_bytes(n, dims, prec), &w))) return rc;
+    char* base = (char*)w;
+    EnsEntry* d_tab = (EnsEntry*)base;
+    base += align256(sizeof(EnsEntry) * n);
+    void* W2 = base;
+    base += align256((size_t)p.field_reals * sz);
+    double* acc = (double*)base;
+    base += align256((size_t)p.acc_doubles * 8);
+    double* partials = (double*)base;
+    hipStream_t st = S(stream);
+    FLUE_TRY(hipMemcpyAsync(d_tab, p.tab.data(), sizeof(EnsEntry) * n, hipMemcpyHostToDevice, st));
+    FLUE_TRY(hipStreamSynchronize(st));  // the host table goes out of scope with this call
+
+    int dev = 0, cus = 0;
+    FLUE_TRY(hipGetDevice(&dev));
+    FLUE_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(p.total_blocks, (uint32_t)std::max(cus, 1) * 4));
+
+    const int nmeas = measure_every > 0 ? nsteps / measure_every + 1 : 0;
+    static const double ca[3] = {0.0, -17.0 / 9.0, -1.0};
+    static const double cb[3] = {0.25, 8.0 / 9.0, 0.75};
+    void* bufs[2] = {fields, W2};
+    auto measure = [&](const void* fld, int m) -> hipError_t {
+        hipError_t e = launch_ens_sum(d_tab, n, p.total_blocks, p.maxV, fld, is_f32, 0, partials, grid, st);
+        if (e == hipSuccess) e = launch_ens_finalize(d_tab, n, partials, d_plaq + (size_t)m * n, 1, 0, st);
+        if (e == hipSuccess) e = launch_ens_sum(d_tab, n, p.total_blocks, p.maxV, fld, is_f32, 1, partials, grid, st);
+        if (e == hipSuccess) e = launch_ens_finalize(d_tab, n, partials, d_B + (size_t)m * n, 1, 1, st);
+        return e;
+    };
+    // the whole run: measurement at step 0 and after every measure_every steps
+    auto run = [&]() -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (nmeas > 0) e = measure(fields, 0);
+        for (int32_t j = 0; j < nsteps && e == hipSuccess; ++j) {
+            for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+                const int q = 3 * j + k;
+                e = launch_ens_flow_stage(d_tab, n, p.total_blocks, p.maxV, bufs[q & 1], is_f32, bufs[(q + 1) & 1], acc,
+                                          eps, ca[k], cb[k], k == 2, grid, st);
+            }
+            if (e == hipSuccess && nmeas > 0 && (j + 1) % measure_every == 0)
+                e = measure(bufs[(j + 1) & 1], (j + 1) / measure_every);
+        }
+        if (e == hipSuccess && (nsteps & 1)) e = launch_copy((size_t)p.field_reals, is_f32, W2, fields, st);
+        return e;
+    };
+    if (use_graph) {
+        if (st == nullptr) return FLUE_EINVAL;  // capture needs a non-default stream
+        FLUE_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = run();
+        hipGraph_t graph = nullptr;
+        hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return hip_status(e != hipSuccess ? e : e2);
+        }
+        hipGraphExec_t exec;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) {
+            e = hipGraphLaunch(exec, st);
+            e2 = hipStreamSynchronize(st);
+            (void)hipGraphExecDestroy(exec);
+        }
+        (void)hipGraphDestroy(graph);
+        FLUE_TRY(e);
+        FLUE_TRY(e2);
+    } else {
+        FLUE_TRY(run());
     }
     return FLUE_OK;
 }

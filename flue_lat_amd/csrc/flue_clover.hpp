@@ -229,6 +229,39 @@ __device__ __forceinline__ AH<typename FV::real> plane_ah(const PlaneGeom<FV>& g
     return h;
 }
 
+// sum_ab |F_ab|^2 = -ReTr(F F) for F = A^T/4 with A the accumulated anti-Hermitian part
+// (FP64): |F_aa|^2 = d_a^2/16 and |F_ab|^2 + |F_ba|^2 = |o_ab|^2/32
+template <typename R>
+__device__ __forceinline__ double f_norm2(const AH<R>& h) {
+    double dsum = 0.0, osum = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double d = (double)h.d[k];
+        const double orr = (double)h.or_[k], oi = (double)h.oi[k];
+        dsum += d * d;
+        osum += orr * orr + oi * oi;
+    }
+    return (dsum + 0.5 * osum) * (1.0 / 16.0);
+}
+
+// -ReTr F_i^2 summed over the three magnetic planes (y,x), (x,z), (z,y) at site s
+// (src/wloops.F90:502-506, 530-533)
+template <typename FV>
+__device__ __forceinline__ double magnetic_site(const Lat& L, const FV& f, uint32_t s) {
+    int c[4];
+    site_coords(L, s, c);
+    double acc = 0.0;
+#pragma unroll 1
+    for (int pl = 0; pl < 3; ++pl) {
+        const int mu = pl == 0 ? 2 : (pl == 1 ? 1 : 3);
+        const int nu = pl == 0 ? 1 : (pl == 1 ? 3 : 2);
+        const int cmu = pl == 0 ? c[2] : (pl == 1 ? c[1] : c[3]);
+        const int cnu = pl == 0 ? c[1] : (pl == 1 ? c[3] : c[2]);
+        acc += f_norm2(plane_ah<false>(plane_geom(L, f, s, cmu, cnu, mu, nu)));
+    }
+    return acc;
+}
+
 
 }  // namespace
 }  // namespace flue

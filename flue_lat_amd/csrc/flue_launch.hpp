@@ -53,4 +53,23 @@ hipError_t launch_b2_grad(const Lat& L, const void* U, int is_f32, const void* A
 hipError_t launch_finalize_affine(const double* partials, uint32_t n, double* out, double scale, double offset,
                                   hipStream_t st);
 
+// ensembles (k_ensemble.hip): configuration c's field starts at element `off` of the
+// back-to-back fields, its FP64 flow accumulator at `acc_off`, and it owns logical
+// blocks [blk0, blk0 + nblk) of 256 sites
+struct EnsEntry {
+    Lat L;
+    uint64_t off;
+    uint64_t acc_off;
+    uint32_t blk0;
+    uint32_t nblk;
+};
+hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* win,
+                                 int is_f32, void* wout, double* acc, double eps, double ca, double cb, int reunit,
+                                 uint32_t grid, hipStream_t st);
+hipError_t launch_ens_sum(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* fld, int is_f32,
+                          int obs, double* partials, uint32_t grid, hipStream_t st);
+hipError_t launch_ens_finalize(const EnsEntry* tab, int n, const double* partials, double* out, int stride,
+                               int per_volume, hipStream_t st);
+hipError_t launch_copy(size_t n_reals, int is_f32, const void* src, void* dst, hipStream_t st);
+
 }  // namespace flue

@@ -8,20 +8,6 @@ namespace flue {
 
 namespace {
 
-// sum_ab |F_ab|^2 = -ReTr(F F) for F = A^T/4 with A = AH part (all in FP64)
-template <typename R>
-__device__ __forceinline__ double f_norm2(const AH<R>& h) {
-    double dsum = 0.0, osum = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double d = (double)h.d[k];
-        const double orr = (double)h.or_[k], oi = (double)h.oi[k];
-        dsum += d * d;
-        osum += orr * orr + oi * oi;
-    }
-    return (dsum + 0.5 * osum) * (1.0 / 16.0);
-}
-
 constexpr int kBS = 256;
 
 // B = -(1/V) sum_x sum_i ReTr(F_i F_i) over the planes (y,x), (x,z), (z,y)

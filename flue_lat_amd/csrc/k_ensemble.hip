@@ -1,0 +1,156 @@
+// k_ensemble.hip -- ensembles of gauge fields with different extents processed in one
+// launch per operation: the RK3 flow stage, the plaquette sum and the magnetic B.
+//
+// The fields lie back to back, each in the SoA layout of its own lattice.  Every
+// configuration owns a contiguous range of logical blocks (256 sites each); a persistent
+// grid walks the logical blocks in contiguous chunks per workgroup (chunks of one XCD
+// adjacent), finds the configuration of a logical block by binary search in a table
+// kept in device memory, and runs the same per-site code as the single-field kernels.
+// Sums are reduced per logical block and then per configuration in a fixed order.
+#include "flue_clover.hpp"
+#include "flue_flow.hpp"
+#include "flue_launch.hpp"
+
+namespace flue {
+
+namespace {
+
+constexpr int kBS = 256;
+
+__device__ __forceinline__ int find_cfg(const EnsEntry* __restrict__ tab, int n, uint32_t lb) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].blk0 <= lb)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// logical blocks [lo, hi) of this workgroup
+__device__ __forceinline__ void my_range(uint32_t total, uint32_t& lo, uint32_t& hi) {
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    lo = (uint32_t)(((uint64_t)total * b) / gridDim.x);
+    hi = (uint32_t)(((uint64_t)total * (b + 1)) / gridDim.x);
+}
+
+template <typename R, bool BUF>
+__global__ void __launch_bounds__(kBS) ens_flow_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
+                                                       const R* __restrict__ win, R* __restrict__ wout,
+                                                       double* __restrict__ acc, double eps, double ca, double cb,
+                                                       int reunit) {
+    uint32_t lo, hi;
+    my_range(total, lo, hi);
+    for (uint32_t lb = lo; lb < hi; ++lb) {
+        const EnsEntry e = tab[find_cfg(tab, n, lb)];
+        const uint32_t s = (lb - e.blk0) * kBS + threadIdx.x;
+        if (s < e.L.V) {
+            const FieldView<R, BUF> f{win + e.off, e.L.V};
+            flow_site(e.L, f, wout + e.off, acc + e.acc_off, s, eps, ca, cb, reunit);
+        }
+    }
+}
+
+// OBS = 0: sum of ReTr over all 6V plaquettes; OBS = 1: sum of -ReTr F_i^2 (magnetic)
+template <typename R, bool BUF, int OBS>
+__global__ void __launch_bounds__(kBS) ens_sum_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
+                                                      const R* __restrict__ fld, double* __restrict__ partials) {
+    uint32_t lo, hi;
+    my_range(total, lo, hi);
+    for (uint32_t lb = lo; lb < hi; ++lb) {
+        const EnsEntry e = tab[find_cfg(tab, n, lb)];
+        const uint32_t s = (lb - e.blk0) * kBS + threadIdx.x;
+        double v = 0.0;
+        if (s < e.L.V) {
+            const FieldView<R, BUF> f{fld + e.off, e.L.V};
+            v = OBS == 0 ? plaquette_site(e.L, f, s, 0, 3, 3) : magnetic_site(e.L, f, s);
+        }
+        const DD r = block_sum<kBS>(DD{v, 0.0});
+        if (threadIdx.x == 0) {
+            partials[2 * lb] = r.hi;
+            partials[2 * lb + 1] = r.lo;
+        }
+        __syncthreads();  // block_sum's shared slots are reused by the next logical block
+    }
+}
+
+// one workgroup per configuration: out[c * stride] = scale_c * sum of its block partials
+__global__ void __launch_bounds__(kBS) ens_finalize_kernel(const EnsEntry* __restrict__ tab,
+                                                           const double* __restrict__ partials,
+                                                           double* __restrict__ out, int stride, int per_volume) {
+    const EnsEntry e = tab[blockIdx.x];
+    const uint32_t lo = e.blk0 + (uint32_t)(((uint64_t)e.nblk * threadIdx.x) / kBS);
+    const uint32_t hi = e.blk0 + (uint32_t)(((uint64_t)e.nblk * (threadIdx.x + 1)) / kBS);
+    DD acc = {0.0, 0.0};
+    for (uint32_t i = lo; i < hi; ++i) acc = dd_add(acc, DD{partials[2 * i], partials[2 * i + 1]});
+    const DD r = block_sum<kBS>(acc);
+    if (threadIdx.x == 0) {
+        const double v = r.hi + r.lo;
+        out[(size_t)blockIdx.x * stride] = per_volume ? v / (double)e.L.V : v;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBS) copy_kernel(size_t n, const R* __restrict__ src, R* __restrict__ dst) {
+    for (size_t i = (size_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (size_t)gridDim.x * kBS) dst[i] = src[i];
+}
+
+template <typename Fn>
+void with_real(int is_f32, bool buf, Fn&& fn) {
+    if (is_f32) {
+        if (buf)
+            fn(float(0), std::true_type());
+        else
+            fn(float(0), std::false_type());
+    } else {
+        if (buf)
+            fn(double(0), std::true_type());
+        else
+            fn(double(0), std::false_type());
+    }
+}
+
+}  // namespace
+
+hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* win,
+                                 int is_f32, void* wout, double* acc, double eps, double ca, double cb, int reunit,
+                                 uint32_t grid, hipStream_t st) {
+    with_real(is_f32, fits_buffer(maxV, is_f32), [&](auto r, auto buf) {
+        using R = decltype(r);
+        ens_flow_kernel<R, decltype(buf)::value><<<grid, kBS, 0, st>>>(tab, n, total, (const R*)win, (R*)wout, acc,
+                                                                      eps, ca, cb, reunit);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_ens_sum(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* fld, int is_f32,
+                          int obs, double* partials, uint32_t grid, hipStream_t st) {
+    with_real(is_f32, fits_buffer(maxV, is_f32), [&](auto r, auto buf) {
+        using R = decltype(r);
+        constexpr bool B = decltype(buf)::value;
+        if (obs == 0)
+            ens_sum_kernel<R, B, 0><<<grid, kBS, 0, st>>>(tab, n, total, (const R*)fld, partials);
+        else
+            ens_sum_kernel<R, B, 1><<<grid, kBS, 0, st>>>(tab, n, total, (const R*)fld, partials);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_ens_finalize(const EnsEntry* tab, int n, const double* partials, double* out, int stride,
+                               int per_volume, hipStream_t st) {
+    ens_finalize_kernel<<<n, kBS, 0, st>>>(tab, partials, out, stride, per_volume);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(size_t n_reals, int is_f32, const void* src, void* dst, hipStream_t st) {
+    const uint32_t g = (uint32_t)std::min<size_t>((n_reals + kBS - 1) / kBS, 8192);
+    if (is_f32)
+        copy_kernel<float><<<g, kBS, 0, st>>>(n_reals, (const float*)src, (float*)dst);
+    else
+        copy_kernel<double><<<g, kBS, 0, st>>>(n_reals, (const double*)src, (double*)dst);
+    return hipGetLastError();
+}
+
+}  // namespace flue

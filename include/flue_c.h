@@ -229,6 +229,27 @@ int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* 
 int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, int32_t nsteps, int32_t use_graph,
                      void* work, size_t work_bytes, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* 4. ensembles: n fields with their own extents (dims is n x {NT,NX,NY,NZ}) stored     */
+/*    back to back in `fields`, each in the SoA layout, one launch per operation       */
+/* ------------------------------------------------------------------ */
+size_t flue_ensemble_field_bytes(int32_t n, const int32_t* dims, int32_t prec);
+size_t flue_ensemble_work_bytes(int32_t n, const int32_t* dims, int32_t prec);
+
+/* field c from the seeded generator with seeds[c] (see flue_random_field) */
+int flue_ensemble_random(int32_t n, const int32_t* dims, const uint64_t* seeds, double eps, void* fields,
+                         int32_t prec, void* stream);
+
+/* nsteps RK3 Wilson-flow steps of every field in place.  With measure_every > 0 the
+ * plaquette sum (sum of ReTr P over the 6V plaquettes, as genPlaquette(1,4,4)) and the
+ * magnetic B are written to d_plaq[m*n + c] and d_B[m*n + c] at steps
+ * m*measure_every, m = 0 .. nsteps/measure_every.  use_graph != 0 captures the whole run
+ * in one hipGraph and launches it (needs a non-default stream); the call returns after
+ * the run completes in that case. */
+int flue_ensemble_flow(int32_t n, const int32_t* dims, void* fields, int32_t prec, double eps, int32_t nsteps,
+                       int32_t measure_every, double* d_plaq, double* d_B, int32_t use_graph, void* work,
+                       size_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
