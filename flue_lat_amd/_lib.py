@@ -80,6 +80,10 @@ SIGNATURES = {
     "flue_b2_grad": [_P, _P, _I, _P, _P, _SZ, _P],
     "flue_flow_stage": [_P, _P, _I, _P, _P, _D, _I, _P],
     "flue_wilson_flow": [_P, _P, _I, _D, _I, _I, _P, _SZ, _P],
+    "flue_autotune": [_I, _P, _P, _I, _I, _P, _P, _P, _SZ, _P],
+    "flue_tuned_variant": [_I, _P, _I, _P],
+    "flue_tune_save": [ctypes.c_char_p],
+    "flue_tune_load": [ctypes.c_char_p],
     "flue_ensemble_field_bytes": [_I, _P, _I],
     "flue_ensemble_work_bytes": [_I, _P, _I],
     "flue_ensemble_random": [_I, _P, _P, _D, _P, _I, _P],

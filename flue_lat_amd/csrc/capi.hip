@@ -2,7 +2,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -80,6 +82,38 @@ int get_ws(void* ws, size_t ws_bytes, size_t need, void** out) {
 }
 
 size_t partial_bytes(uint32_t nblocks) { return (size_t)nblocks * 2 * sizeof(double) + 256; }
+
+// ---------------------------------------------------------------- autotuner table
+// (device, op, precision, NT, NX, NY, NZ) -> fastest kernel variant measured on it
+struct TuneKey {
+    int dev, op, prec, d[4];
+    bool operator<(const TuneKey& o) const {
+        if (dev != o.dev) return dev < o.dev;
+        if (op != o.op) return op < o.op;
+        if (prec != o.prec) return prec < o.prec;
+        for (int i = 0; i < 4; ++i)
+            if (d[i] != o.d[i]) return d[i] < o.d[i];
+        return false;
+    }
+};
+struct TuneVal {
+    int variant;
+    double ms;
+};
+std::mutex g_tune_mutex;
+std::map<TuneKey, TuneVal> g_tune;
+
+TuneKey tune_key(int op, const int32_t dims[4], int prec) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return TuneKey{dev, op, prec, {dims[0], dims[1], dims[2], dims[3]}};
+}
+
+int tuned_variant(int op, const int32_t dims[4], int prec) {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    auto it = g_tune.find(tune_key(op, dims, prec));
+    return it == g_tune.end() ? 0 : it->second.variant;
+}
 
 hipStream_t S(void* s) { return (hipStream_t)s; }
 
@@ -389,6 +423,7 @@ int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t
     void* w;
     if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
     if (variant < 0 || variant > 3) return FLUE_EINVAL;
+    if (variant == 0) variant = tuned_variant(FLUE_OP_MAGNETIC, dims, prec);
     FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, (double*)w, S(stream), variant));
     FLUE_TRY(launch_finalize((double*)w, nb, d_B, 1.0 / (double)L.V, S(stream)));
     return FLUE_OK;
@@ -522,6 +557,98 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
         FLUE_TRY(hipMemcpyAsync(U, W2, (size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8), hipMemcpyDeviceToDevice,
                                 st));
     }
+    return FLUE_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+
+int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec, int32_t reps, int32_t* best,
+                  double* best_ms, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U) return FLUE_ENULL;
+    if (op != FLUE_OP_MAGNETIC || check_prec(prec) || reps < 1) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    const int is_f32 = prec == FLUE_F32;
+    if (!fits_buffer(L.V, is_f32)) {
+        // only one variant exists for fields beyond the buffer-resource range
+        if (best) *best = magnetic_default_variant(is_f32);
+        if (best_ms) *best_ms = -1.0;
+        return FLUE_OK;
+    }
+    const uint32_t nb = magnetic_blocks(L, is_f32);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
+    hipStream_t st = S(stream);
+    hipEvent_t e0, e1;
+    FLUE_TRY(hipEventCreate(&e0));
+    FLUE_TRY(hipEventCreate(&e1));
+    int bv = 0;
+    float bms = 0.f;
+    hipError_t e = hipSuccess;
+    for (int v = 1; v <= 3 && e == hipSuccess; ++v) {
+        e = launch_magnetic(L, U, is_f32, (double*)w, st, v);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_magnetic(L, U, is_f32, (double*)w, st, v);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && (bv == 0 || ms < bms)) {
+            bv = v;
+            bms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    FLUE_TRY(e);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_tune[tune_key(op, dims, prec)] = TuneVal{bv, (double)bms / reps};
+    }
+    if (best) *best = bv;
+    if (best_ms) *best_ms = (double)bms / reps;
+    return FLUE_OK;
+}
+
+int flue_tuned_variant(int32_t op, const int32_t dims[4], int32_t prec, int32_t* variant) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!variant) return FLUE_ENULL;
+    if (check_prec(prec)) return FLUE_EINVAL;
+    *variant = tuned_variant(op, dims, prec);
+    return FLUE_OK;
+}
+
+int flue_tune_save(const char* path) {
+    if (!path) return FLUE_ENULL;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return FLUE_EINVAL;
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    std::fprintf(f, "# flue autotune: device op prec NT NX NY NZ variant ms_per_call\n");
+    for (const auto& kv : g_tune)
+        std::fprintf(f, "%d %d %d %d %d %d %d %d %.6f\n", kv.first.dev, kv.first.op, kv.first.prec, kv.first.d[0],
+                     kv.first.d[1], kv.first.d[2], kv.first.d[3], kv.second.variant, kv.second.ms);
+    std::fclose(f);
+    return FLUE_OK;
+}
+
+int flue_tune_load(const char* path) {
+    if (!path) return FLUE_ENULL;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return FLUE_EINVAL;
+    char line[256];
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        TuneKey k;
+        TuneVal v;
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %lf", &k.dev, &k.op, &k.prec, &k.d[0], &k.d[1], &k.d[2],
+                        &k.d[3], &v.variant, &v.ms) == 9 &&
+            v.variant >= 1 && v.variant <= 3)
+            g_tune[k] = v;
+    }
+    std::fclose(f);
     return FLUE_OK;
 }
 

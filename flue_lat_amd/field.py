@@ -48,6 +48,16 @@ def require_gpu():
         raise RuntimeError("flue_lat_amd device path needs a visible GPU")
 
 
+def tune_save(path):
+    """Write the autotuner's table (fastest kernel variant per device, op, precision, extents)."""
+    check(_lib.lib().flue_tune_save(str(path).encode()), "flue_tune_save")
+
+
+def tune_load(path):
+    """Merge a table written by tune_save into the autotuner."""
+    check(_lib.lib().flue_tune_load(str(path).encode()), "flue_tune_load")
+
+
 class GaugeField:
     """SU(3) gauge field on the GPU in the SoA layout.
 
@@ -197,6 +207,19 @@ class GaugeField:
             check(_lib.lib().flue_magnetic_ex(self._cdims, _ptr(self.data), self.prec, int(variant), _ptr(out),
                                               _ptr(ws), ws.numel(), _stream()), "flue_magnetic_ex")
         return out.item() if sync else out
+
+    def autotune(self, op="magnetic", reps=5):
+        """Time the kernel variants of `op` on this field and remember the fastest for its
+        (device, precision, extents); returns (variant, ms per call)."""
+        ops = {"magnetic": 0}
+        best = ctypes.c_int32()
+        ms = ctypes.c_double()
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_autotune(ops[op], self._cdims, _ptr(self.data), self.prec, int(reps),
+                                           ctypes.byref(best), ctypes.byref(ms), _ptr(ws), ws.numel(), _stream()),
+                  "flue_autotune")
+        return best.value, ms.value
 
     # ------------------------------------------------------------ gradients and flow
     def _like(self):

@@ -230,6 +230,18 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
                      void* work, size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------ */
+/* autotuner: kernels with several variants (occupancy targets) are timed with hipEvents   */
+/* on the caller's field; the fastest is remembered per (device, op, precision, extents)  */
+/* and used whenever the variant argument is 0.  The table persists through save/load.    */
+/* ------------------------------------------------------------------ */
+enum { FLUE_OP_MAGNETIC = 0 };
+int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec, int32_t reps, int32_t* best,
+                  double* best_ms, void* ws, size_t ws_bytes, void* stream);
+int flue_tuned_variant(int32_t op, const int32_t dims[4], int32_t prec, int32_t* variant);
+int flue_tune_save(const char* path);
+int flue_tune_load(const char* path);
+
+/* ------------------------------------------------------------------ */
 /* 4. ensembles: n fields with their own extents (dims is n x {NT,NX,NY,NZ}) stored     */
 /*    back to back in `fields`, each in the SoA layout, one launch per operation       */
 /* ------------------------------------------------------------------ */

@@ -98,3 +98,18 @@ def test_no_gpu_is_reported_not_faked(L):
     with pytest.raises(FlueError) as e:
         FL.compiled.flue_c.magnetic_c(np.zeros((3, 3, 3, 3, 4, 3, 3), complex))
     assert e.value.status == 9
+
+
+def test_autotune_table_roundtrip(L, tmp_path):
+    import ctypes
+    p = tmp_path / "tune.txt"
+    p.write_text("# header\n0 0 1 4 5 6 7 3 0.25\n0 0 0 4 5 6 7 9 0.5\n")
+    assert L.flue_tune_load(str(p).encode()) == 0
+    v = ctypes.c_int32(-1)
+    d = (ctypes.c_int32 * 4)(4, 5, 6, 7)
+    assert L.flue_tuned_variant(0, d, 1, ctypes.byref(v)) == 0 and v.value == 3
+    # out-of-range variants are ignored on load
+    assert L.flue_tuned_variant(0, d, 0, ctypes.byref(v)) == 0 and v.value == 0
+    q = tmp_path / "out.txt"
+    assert L.flue_tune_save(str(q).encode()) == 0
+    assert "0 0 1 4 5 6 7 3" in q.read_text()

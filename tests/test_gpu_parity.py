@@ -233,3 +233,12 @@ def test_wloops_module_accepts_all_field_kinds(FL, small):
     s, n, _ = W.genPlaquette(torch.from_numpy(small), 2, 4, 4)
     ws, wn = O.gen_plaquette(small, 2, 4, 4)
     assert n == wn and rel(s, ws) < 1e-12
+
+
+def test_autotune_picks_a_variant_and_keeps_results(FL, c0):
+    g = gfield(FL, c0, "fp32")
+    B0 = g.magnetic(variant=2)
+    v, ms = g.autotune("magnetic", reps=2)
+    assert v in (1, 2, 3) and ms > 0
+    assert g.magnetic() == g.magnetic(variant=v)
+    assert abs(g.magnetic() - B0) <= 1e-12 * B0
