@@ -14,6 +14,7 @@ Prints one JSON line.  Only one GPU is used: --gpus N > 1 exits with a message.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,7 +31,7 @@ FLOP_PER_SITE = 258 * 198
 BYTES_PER_SITE = {"fp32": 3 * 72, "fp64": 3 * 144}
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (FP32 counts packed FMA)
 PEAK_HBM_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "magnetic_fp32_48x96_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "magnetic_48x96_traffic.md")
 
 
 def time_steps(fn, steps, warmup, torch):
@@ -49,21 +50,36 @@ def time_steps(fn, steps, warmup, torch):
     return ev0.elapsed_time(ev1) / 1e3 / steps, wall / steps
 
 
-def cpu_baseline(nslices=4):
-    """The CPU oracle (FP64, OpenMP) on a bounded sample of the same workload: nslices
-    time slices of a 48^3 x nslices lattice from the same generator (B uses spatial
-    planes only, so each slice is the same work as a slice of the full lattice)."""
+def cpu_baseline(nslices=8, min_seconds=12.0):
+    """The CPU oracle (FP64, OpenMP) on a bounded sample of the same workload: a 48^3 x
+    nslices lattice from the same generator, measured repeatedly until min_seconds of
+    CPU work have passed.  B uses spatial planes only, so each time slice is the same
+    work per site as a slice of the full 48^3x96 lattice."""
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     dims = (nslices,) + DIMS[1:]
     U = O.random_field(dims, SEED, EPS)
+    reps = 0
     t0 = time.perf_counter()
-    O.magnetic_sum_tslices(U, 1, nslices)
-    dt = time.perf_counter() - t0
-    sites = nslices * DIMS[1] * DIMS[2] * DIMS[3]
+    while True:
+        O.magnetic_sum_tslices(U, 1, nslices)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            break
+    sites = reps * nslices * DIMS[1] * DIMS[2] * DIMS[3]
     return {"value": sites / dt / 1e9, "unit": "Gsites/s", "cores": threads, "kind": "port",
-            "sample": f"oracle magnetic (FP64, {threads} OpenMP threads) on {nslices} time slices of 48^3: "
-                      f"{sites} sites in {dt:.2f} s"}
+            "sample": f"oracle magnetic (FP64, {threads} OpenMP threads) over {reps} passes of a "
+                      f"48^3x{nslices} lattice: {sites} sites in {dt:.2f} s"}
+
+
+def hbm_traffic():
+    """HBM bytes per launch of the FP32 magnetic kernel, from the FETCH_SIZE/WRITE_SIZE
+    counter summary kept with the profiles (None when that summary is absent)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    m = re.search(r"^hbm_bytes_per_launch_fp32\s*=\s*(\d+)", open(TRAFFIC_FILE).read(), flags=re.M)
+    return int(m.group(1)) if m else None
 
 
 def main():
@@ -110,10 +126,7 @@ def main():
         extras[name] = {"plaquette": s / (3 * n), "polyakov": p / (3 * npl)}
     rel_B = abs(res["fp32"]["B"] - res["fp64"]["B"]) / abs(res["fp64"]["B"])
 
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    traffic = hbm_traffic()
 
     t32 = res["fp32"]["s_per_step"]
     achieved_tflops = FLOP_PER_SITE * V / t32 / 1e12
