@@ -422,10 +422,11 @@ int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t
     const uint32_t nb = magnetic_blocks(L, prec == FLUE_F32);
     void* w;
     if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
-    if (variant < 0 || variant > 3) return FLUE_EINVAL;
+    if (variant < 0 || variant > kMagneticVariants) return FLUE_EINVAL;
     if (variant == 0) variant = tuned_variant(FLUE_OP_MAGNETIC, dims, prec);
-    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, (double*)w, S(stream), variant));
-    FLUE_TRY(launch_finalize((double*)w, nb, d_B, 1.0 / (double)L.V, S(stream)));
+    uint32_t np = 0;
+    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, (double*)w, S(stream), variant, &np));
+    FLUE_TRY(launch_finalize((double*)w, np, d_B, 1.0 / (double)L.V, S(stream)));
     return FLUE_OK;
 }
 
@@ -586,10 +587,11 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
     int bv = 0;
     float bms = 0.f;
     hipError_t e = hipSuccess;
-    for (int v = 1; v <= 3 && e == hipSuccess; ++v) {
-        e = launch_magnetic(L, U, is_f32, (double*)w, st, v);  // warm-up
+    for (int v = 1; v <= kMagneticVariants && e == hipSuccess; ++v) {
+        e = launch_magnetic(L, U, is_f32, (double*)w, st, v, nullptr);  // warm-up
         if (e == hipSuccess) e = hipEventRecord(e0, st);
-        for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_magnetic(L, U, is_f32, (double*)w, st, v);
+        for (int r = 0; r < reps && e == hipSuccess; ++r)
+            e = launch_magnetic(L, U, is_f32, (double*)w, st, v, nullptr);
         if (e == hipSuccess) e = hipEventRecord(e1, st);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
@@ -645,7 +647,7 @@ int flue_tune_load(const char* path) {
         TuneVal v;
         if (std::sscanf(line, "%d %d %d %d %d %d %d %d %lf", &k.dev, &k.op, &k.prec, &k.d[0], &k.d[1], &k.d[2],
                         &k.d[3], &v.variant, &v.ms) == 9 &&
-            v.variant >= 1 && v.variant <= 3)
+            v.variant >= 1 && v.variant <= kMagneticVariants)
             g_tune[k] = v;
     }
     std::fclose(f);

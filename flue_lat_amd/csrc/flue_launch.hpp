@@ -39,8 +39,10 @@ hipError_t launch_fmunu(const Lat& L, const void* U, int is_f32, int mu, int nu,
                         int out_layout, const int64_t strides[6], hipStream_t st);
 uint32_t magnetic_blocks(const Lat& L, int is_f32);
 int magnetic_default_variant(int is_f32);
-hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st,
-                           int variant = 0);
+// number of magnetic kernel variants (forms x occupancy targets) the autotuner chooses from
+constexpr int kMagneticVariants = 6;
+hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st, int variant,
+                           uint32_t* npart);
 
 // gradients and flow (k_grad.hip, k_b2grad.hip); grad / Wout / Abuf are SoA in the field
 // precision, acc is FP64 (8 su(3) coordinates x 4 directions x V)

@@ -10,31 +10,85 @@ namespace {
 
 constexpr int kBS = 256;
 
+// Two forms of the magnetic kernel, picked per precision (or by the autotuner):
+//
+// site-major: one thread per site does the three planes in turn, sites in storage order.
+//
+// plane-major: the grid is (nb, 3) and blockIdx.y is the plane, so the dispatcher runs
+// the planes one after another.  The blocks of one XCD run through a contiguous range
+// of a per-plane visit order; each order makes the sites in flight whole layers of
+// the two plane directions, stacked along the third spatial direction (rho), so the
+// +-3 reach of the leaves stays inside layers that are already in that XCD's L2:
+//   plane 0 (y,x), rho = z:  x, y, z, t            (the storage order)
+//   plane 1 (x,z), rho = y:  x, z, y, t
+//   plane 2 (z,y), rho = x:  x mod xw, y, z, x / xw, t   (x in chunks of xw sites, at
+//                            least 128 bytes of one element plane, so loads coalesce)
+// This cuts the traffic behind L2 several-fold; it pays where occupancy is low (FP64).
+struct PlaneOrder {
+    uint32_t xw;   // x chunk width for plane 2 (divides NX)
+};
+
+__device__ __forceinline__ uint32_t plane_site(const Lat& L, const PlaneOrder& po, int pl, uint32_t o, int c[4]) {
+    const uint32_t nx = (uint32_t)L.n[1], ny = (uint32_t)L.n[2], nz = (uint32_t)L.n[3];
+    uint32_t q;
+    if (pl == 0) {
+        site_coords(L, o, c);
+    } else if (pl == 1) {
+        c[1] = (int)(o % nx);
+        q = o / nx;
+        c[3] = (int)(q % nz);
+        q /= nz;
+        c[2] = (int)(q % ny);
+        c[0] = (int)(q / ny);
+    } else {
+        const uint32_t xi = o % po.xw;
+        q = o / po.xw;
+        c[2] = (int)(q % ny);
+        q /= ny;
+        c[3] = (int)(q % nz);
+        q /= nz;
+        const uint32_t nxc = nx / po.xw;
+        c[1] = (int)((q % nxc) * po.xw + xi);
+        c[0] = (int)(q / nxc);
+    }
+    return (uint32_t)c[1] + nx * ((uint32_t)c[2] + ny * ((uint32_t)c[3] + nz * (uint32_t)c[0]));
+}
+
+// (mu, nu) of magnetic plane pl: (y, x), (x, z), (z, y) in 0-based directions
+__device__ __forceinline__ int plane_mu(int pl) { return pl == 0 ? 2 : (pl == 1 ? 1 : 3); }
+__device__ __forceinline__ int plane_nu(int pl) { return pl == 0 ? 1 : (pl == 1 ? 3 : 2); }
+
 // B = -(1/V) sum_x sum_i ReTr(F_i F_i) over the planes (y,x), (x,z), (z,y)
-// (src/wloops.F90:502-506, 530-543); this kernel writes one partial sum per block.
-template <typename FV, int WPE>
+// (src/wloops.F90:502-506, 530-543); each block writes one compensated partial sum.
+template <typename FV, int WPE, bool PLANE_MAJOR>
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-magnetic_kernel(Lat L, FV f, double* __restrict__ partials) {
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = blk * kBS + threadIdx.x;
+magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ partials) {
+    // the plane comes from blockIdx.y so that it, and every buffer descriptor derived
+    // from it, stays in scalar registers (a division of blockIdx.x would not)
+    const int pl = (int)blockIdx.y;
+    const uint32_t blk = xcd_block(blockIdx.x, nb);
+    const uint32_t o = blk * kBS + threadIdx.x;
     double acc = 0.0;
-    if (s < L.V) {
+    if (o < L.V) {
         int c[4];
-        site_coords(L, s, c);
+        if constexpr (PLANE_MAJOR) {
+            const uint32_t s = plane_site(L, po, pl, o, c);
+            const int mu = plane_mu(pl), nu = plane_nu(pl);
+            acc = f_norm2(plane_ah<false>(plane_geom(L, f, s, c[mu], c[nu], mu, nu)));
+        } else {
+            site_coords(L, o, c);
 #pragma unroll 1
-        for (int pl = 0; pl < 3; ++pl) {
-            // (mu, nu) = (y, x), (x, z), (z, y) in 0-based directions
-            const int mu = pl == 0 ? 2 : (pl == 1 ? 1 : 3);
-            const int nu = pl == 0 ? 1 : (pl == 1 ? 3 : 2);
-            const int cmu = pl == 0 ? c[2] : (pl == 1 ? c[1] : c[3]);
-            const int cnu = pl == 0 ? c[1] : (pl == 1 ? c[3] : c[2]);
-            acc += f_norm2(plane_ah<false>(plane_geom(L, f, s, cmu, cnu, mu, nu)));
+            for (int p = 0; p < 3; ++p) {
+                const int mu = plane_mu(p), nu = plane_nu(p);
+                acc += f_norm2(plane_ah<false>(plane_geom(L, f, o, c[mu], c[nu], mu, nu)));
+            }
         }
     }
     DD r = block_sum<kBS>(DD{acc, 0.0});
     if (threadIdx.x == 0) {
-        partials[2 * blk] = r.hi;
-        partials[2 * blk + 1] = r.lo;
+        const uint32_t i = (uint32_t)pl * nb + blk;
+        partials[2 * i] = r.hi;
+        partials[2 * i + 1] = r.lo;
     }
 }
 
@@ -90,30 +144,50 @@ fmunu_kernel(Lat L, FV f, int mu, int nu, void* __restrict__ out, int layout, Ou
 
 }  // namespace
 
-uint32_t magnetic_blocks(const Lat& L, int) { return (L.V + kBS - 1) / kBS; }
+uint32_t magnetic_blocks(const Lat& L, int) { return 3 * ((L.V + kBS - 1) / kBS); }
 
-int magnetic_default_variant(int is_f32) { return is_f32 ? 2 : 1; }
+// x chunk width of plane 2: the smallest divisor of NX spanning >= 128 bytes, else NX
+static PlaneOrder plane_order(const Lat& L, size_t real_bytes) {
+    PlaneOrder po{(uint32_t)L.n[1]};
+    for (int w = 1; w < L.n[1]; ++w)
+        if (L.n[1] % w == 0 && (size_t)w * real_bytes >= 128) {
+            po.xw = (uint32_t)w;
+            break;
+        }
+    return po;
+}
 
-// variant = occupancy target in waves per SIMD (1, 2 or 3); 0 picks the default
-hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st, int variant) {
-    const uint32_t g = magnetic_blocks(L, is_f32);
+int magnetic_default_variant(int is_f32) { return is_f32 ? 2 : 4; }
+
+// variant 1..3: site-major with an occupancy target of 1..3 waves per SIMD;
+// variant 4..6: plane-major with 1..3 waves per SIMD; 0 picks the precision's default.
+// *npart receives the number of partial sums written (nb or 3 nb).
+hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st, int variant,
+                           uint32_t* npart) {
+    const uint32_t nb = magnetic_blocks(L, is_f32) / 3;
     if (variant == 0) variant = magnetic_default_variant(is_f32);
-    if (variant < 1 || variant > 3) return hipErrorInvalidValue;
+    if (variant < 1 || variant > kMagneticVariants) return hipErrorInvalidValue;
     with_view(L.V, U, is_f32, [&](auto f) {
         using F = decltype(f);
         using R = typename F::real;
+        const PlaneOrder po = plane_order(L, sizeof(R));
+        const dim3 g1(nb, 1), g3(nb, 3);
         if constexpr (!F::buffered) {
-            // fields beyond the buffer-resource range: one variant only
-            magnetic_kernel<F, sizeof(R) == 4 ? 2 : 1><<<g, kBS, 0, st>>>(L, f, partials);
+            // fields beyond the buffer-resource range: one form only
+            magnetic_kernel<F, sizeof(R) == 4 ? 2 : 1, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials);
+            variant = 1;
         } else {
-            if (variant == 1)
-                magnetic_kernel<F, 1><<<g, kBS, 0, st>>>(L, f, partials);
-            else if (variant == 2)
-                magnetic_kernel<F, 2><<<g, kBS, 0, st>>>(L, f, partials);
-            else
-                magnetic_kernel<F, 3><<<g, kBS, 0, st>>>(L, f, partials);
+            switch (variant) {
+                case 1: magnetic_kernel<F, 1, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                case 2: magnetic_kernel<F, 2, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                case 3: magnetic_kernel<F, 3, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                case 4: magnetic_kernel<F, 1, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                case 5: magnetic_kernel<F, 2, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                default: magnetic_kernel<F, 3, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials); break;
+            }
         }
     });
+    if (npart) *npart = variant >= 4 ? 3 * nb : nb;
     return hipGetLastError();
 }
 

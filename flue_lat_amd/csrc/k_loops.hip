@@ -157,10 +157,9 @@ __global__ void generic_path_kernel(Lat L, FV f, PathArg p, int t, int x, int y,
 // fixed-order compensated sum of per-block partials
 __global__ void __launch_bounds__(kBS) finalize_kernel(const double* __restrict__ partials, uint32_t n,
                                                        double* __restrict__ out, double scale, double offset) {
-    const uint32_t lo = (uint32_t)(((uint64_t)n * threadIdx.x) / kBS);
-    const uint32_t hi = (uint32_t)(((uint64_t)n * (threadIdx.x + 1)) / kBS);
+    // thread t sums entries t, t + kBS, ... in that order (coalesced, fixed order)
     DD acc = {0.0, 0.0};
-    for (uint32_t i = lo; i < hi; ++i) acc = dd_add(acc, DD{partials[2 * i], partials[2 * i + 1]});
+    for (uint32_t i = threadIdx.x; i < n; i += kBS) acc = dd_add(acc, DD{partials[2 * i], partials[2 * i + 1]});
     DD r = block_sum<kBS>(acc);
     if (threadIdx.x == 0) out[0] = offset + (r.hi + r.lo) * scale;
 }

@@ -200,7 +200,8 @@ class GaugeField:
 
     def magnetic(self, sync=True, variant=0):
         """B = -(1/V) sum_x sum_i ReTr F_i^2 over the spatial planes (src/wloops.F90:479-544).
-        variant selects the kernel's occupancy target (0 = library choice)."""
+        variant selects the kernel form and occupancy target: 1-3 site-major, 4-6
+        plane-major, at 1-3 waves per SIMD (0 = tuned or default choice)."""
         out = self._scalar()
         ws = self.workspace()
         with torch.cuda.device(self.device):

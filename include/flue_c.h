@@ -186,8 +186,11 @@ int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, i
 int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_B, void* ws, size_t ws_bytes,
                   void* stream);
 
-/* flue_magnetic with an explicit kernel variant: the occupancy target in waves per
- * SIMD (1, 2 or 3; 0 = the library's choice).  Results do not depend on it. */
+/* flue_magnetic with an explicit kernel variant (0 = the tuned or default choice):
+ *   1..3  site-major (one thread does the three planes of its site), 1..3 waves per SIMD;
+ *   4..6  plane-major (planes run one after another, sites in per-plane L2-friendly
+ *         orders), 1..3 waves per SIMD.
+ * Every variant is deterministic; they agree to rounding (their summation orders differ). */
 int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B, void* ws,
                      size_t ws_bytes, void* stream);
 
