@@ -103,8 +103,8 @@ __device__ __forceinline__ PlaneGeom<FV> plane_geom(const Lat& L, const FV& f, u
     g.f = f;
     g.mu = mu;
     g.nu = nu;
-    const uint32_t smu = L.st[mu], snu = L.st[nu];
-    const int nmu = L.n[mu], nnu = L.n[nu];
+    const uint32_t smu = L.stride(mu), snu = L.stride(nu);
+    const int nmu = L.ext(mu), nnu = L.ext(nu);
     g.sp = s - (uint32_t)cmu * smu - (uint32_t)cnu * snu;
 #pragma unroll
     for (int k = -3; k <= 3; ++k) {

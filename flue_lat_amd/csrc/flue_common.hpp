@@ -22,6 +22,14 @@ struct Lat {
     uint32_t st[4];   // site-index stride per direction
     uint32_t V;       // number of sites
     uint32_t Vs;      // sites per time slice
+
+    // extent and stride for a run-time direction mu
+    __host__ __device__ __forceinline__ int ext(int mu) const {
+        return mu == 0 ? n[0] : (mu == 1 ? n[1] : (mu == 2 ? n[2] : n[3]));
+    }
+    __host__ __device__ __forceinline__ uint32_t stride(int mu) const {
+        return mu == 0 ? st[0] : (mu == 1 ? st[1] : (mu == 2 ? st[2] : st[3]));
+    }
 };
 
 inline Lat make_lat(const int32_t dims_tx_yz[4]) {
@@ -314,20 +322,20 @@ __device__ __forceinline__ void site_coords(const Lat& L, uint32_t s, int c[4]) 
 
 // site reached from s (coords c) by one step along +mu or -mu
 __device__ __forceinline__ uint32_t step_fwd(const Lat& L, uint32_t s, int* c, int mu) {
-    if (c[mu] == L.n[mu] - 1) {
+    if (c[mu] == L.ext(mu) - 1) {
         c[mu] = 0;
-        return s - (uint32_t)(L.n[mu] - 1) * L.st[mu];
+        return s - (uint32_t)(L.ext(mu) - 1) * L.stride(mu);
     }
     c[mu] += 1;
-    return s + L.st[mu];
+    return s + L.stride(mu);
 }
 __device__ __forceinline__ uint32_t step_bwd(const Lat& L, uint32_t s, int* c, int mu) {
     if (c[mu] == 0) {
-        c[mu] = L.n[mu] - 1;
-        return s + (uint32_t)(L.n[mu] - 1) * L.st[mu];
+        c[mu] = L.ext(mu) - 1;
+        return s + (uint32_t)(L.ext(mu) - 1) * L.stride(mu);
     }
     c[mu] -= 1;
-    return s - L.st[mu];
+    return s - L.stride(mu);
 }
 
 // Hardware hands consecutive workgroups to the 8 XCDs in turn; give each XCD a

@@ -90,37 +90,37 @@ struct PlaneWalker {
 
 __device__ __forceinline__ void pw_step(const Lat& L, PlaneWalker& w, int ax_mu, int ax_nu, int di, int dj) {
     if (di > 0) {
-        if (w.ci == L.n[ax_mu] - 1) {
+        if (w.ci == L.ext(ax_mu) - 1) {
             w.ci = 0;
-            w.s -= (uint32_t)(L.n[ax_mu] - 1) * L.st[ax_mu];
+            w.s -= (uint32_t)(L.ext(ax_mu) - 1) * L.stride(ax_mu);
         } else {
             ++w.ci;
-            w.s += L.st[ax_mu];
+            w.s += L.stride(ax_mu);
         }
     } else if (di < 0) {
         if (w.ci == 0) {
-            w.ci = L.n[ax_mu] - 1;
-            w.s += (uint32_t)(L.n[ax_mu] - 1) * L.st[ax_mu];
+            w.ci = L.ext(ax_mu) - 1;
+            w.s += (uint32_t)(L.ext(ax_mu) - 1) * L.stride(ax_mu);
         } else {
             --w.ci;
-            w.s -= L.st[ax_mu];
+            w.s -= L.stride(ax_mu);
         }
     }
     if (dj > 0) {
-        if (w.cj == L.n[ax_nu] - 1) {
+        if (w.cj == L.ext(ax_nu) - 1) {
             w.cj = 0;
-            w.s -= (uint32_t)(L.n[ax_nu] - 1) * L.st[ax_nu];
+            w.s -= (uint32_t)(L.ext(ax_nu) - 1) * L.stride(ax_nu);
         } else {
             ++w.cj;
-            w.s += L.st[ax_nu];
+            w.s += L.stride(ax_nu);
         }
     } else if (dj < 0) {
         if (w.cj == 0) {
-            w.cj = L.n[ax_nu] - 1;
-            w.s += (uint32_t)(L.n[ax_nu] - 1) * L.st[ax_nu];
+            w.cj = L.ext(ax_nu) - 1;
+            w.s += (uint32_t)(L.ext(ax_nu) - 1) * L.stride(ax_nu);
         } else {
             --w.cj;
-            w.s -= L.st[ax_nu];
+            w.s -= L.stride(ax_nu);
         }
     }
 }

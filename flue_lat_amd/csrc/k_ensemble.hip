@@ -17,6 +17,17 @@ namespace {
 
 constexpr int kBS = 256;
 
+// The table is read through the constant address space and its entries are used in
+// place: uniform loads from it become scalar loads, so the lattice geometry and every
+// buffer descriptor built from it stay in scalar registers.  (A local copy of an entry
+// is indexed at run time by direction, lives in memory, and turns every link load into
+// a loop over the distinct descriptors of the wave.)
+using CEnsEntry = const __attribute__((address_space(4))) EnsEntry;
+
+__device__ __forceinline__ const EnsEntry& entry(const EnsEntry* tab, int c) {
+    return *(const EnsEntry*)((CEnsEntry*)tab + c);
+}
+
 __device__ __forceinline__ int find_cfg(const EnsEntry* __restrict__ tab, int n, uint32_t lb) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -29,22 +40,26 @@ __device__ __forceinline__ int find_cfg(const EnsEntry* __restrict__ tab, int n,
     return lo;
 }
 
-// logical blocks [lo, hi) of this workgroup
-__device__ __forceinline__ void my_range(uint32_t total, uint32_t& lo, uint32_t& hi) {
+// logical blocks [lo, hi) of this workgroup: chunks of `chunk` blocks (chunk = total /
+// gridDim rounded up, from the host).  No division here: a division would be lowered to
+// vector instructions, and every value derived from it -- the configuration entry and
+// each buffer descriptor -- would then leave scalar registers.
+__device__ __forceinline__ void my_range(uint32_t total, uint32_t chunk, uint32_t& lo, uint32_t& hi) {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    lo = (uint32_t)(((uint64_t)total * b) / gridDim.x);
-    hi = (uint32_t)(((uint64_t)total * (b + 1)) / gridDim.x);
+    const uint64_t l = (uint64_t)b * chunk;
+    lo = l < total ? (uint32_t)l : total;
+    hi = total - lo > chunk ? lo + chunk : total;
 }
 
 template <typename R, bool BUF>
 __global__ void __launch_bounds__(kBS) ens_flow_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
-                                                       const R* __restrict__ win, R* __restrict__ wout,
+                                                       uint32_t chunk, const R* __restrict__ win, R* __restrict__ wout,
                                                        double* __restrict__ acc, double eps, double ca, double cb,
                                                        int reunit) {
     uint32_t lo, hi;
-    my_range(total, lo, hi);
+    my_range(total, chunk, lo, hi);
     for (uint32_t lb = lo; lb < hi; ++lb) {
-        const EnsEntry e = tab[find_cfg(tab, n, lb)];
+        const EnsEntry& e = entry(tab, find_cfg(tab, n, lb));
         const uint32_t s = (lb - e.blk0) * kBS + threadIdx.x;
         if (s < e.L.V) {
             const FieldView<R, BUF> f{win + e.off, e.L.V};
@@ -56,11 +71,11 @@ __global__ void __launch_bounds__(kBS) ens_flow_kernel(const EnsEntry* __restric
 // OBS = 0: sum of ReTr over all 6V plaquettes; OBS = 1: sum of -ReTr F_i^2 (magnetic)
 template <typename R, bool BUF, int OBS>
 __global__ void __launch_bounds__(kBS) ens_sum_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
-                                                      const R* __restrict__ fld, double* __restrict__ partials) {
+                                                      uint32_t chunk, const R* __restrict__ fld, double* __restrict__ partials) {
     uint32_t lo, hi;
-    my_range(total, lo, hi);
+    my_range(total, chunk, lo, hi);
     for (uint32_t lb = lo; lb < hi; ++lb) {
-        const EnsEntry e = tab[find_cfg(tab, n, lb)];
+        const EnsEntry& e = entry(tab, find_cfg(tab, n, lb));
         const uint32_t s = (lb - e.blk0) * kBS + threadIdx.x;
         double v = 0.0;
         if (s < e.L.V) {
@@ -119,21 +134,23 @@ hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uin
                                  uint32_t grid, hipStream_t st) {
     with_real(is_f32, fits_buffer(maxV, is_f32), [&](auto r, auto buf) {
         using R = decltype(r);
-        ens_flow_kernel<R, decltype(buf)::value><<<grid, kBS, 0, st>>>(tab, n, total, (const R*)win, (R*)wout, acc,
-                                                                      eps, ca, cb, reunit);
+        ens_flow_kernel<R, decltype(buf)::value><<<grid, kBS, 0, st>>>(tab, n, total, (total + grid - 1) / grid,
+                                                                      (const R*)win, (R*)wout, acc, eps, ca, cb,
+                                                                      reunit);
     });
     return hipGetLastError();
 }
 
 hipError_t launch_ens_sum(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* fld, int is_f32,
                           int obs, double* partials, uint32_t grid, hipStream_t st) {
+    const uint32_t chunk = (total + grid - 1) / grid;
     with_real(is_f32, fits_buffer(maxV, is_f32), [&](auto r, auto buf) {
         using R = decltype(r);
         constexpr bool B = decltype(buf)::value;
         if (obs == 0)
-            ens_sum_kernel<R, B, 0><<<grid, kBS, 0, st>>>(tab, n, total, (const R*)fld, partials);
+            ens_sum_kernel<R, B, 0><<<grid, kBS, 0, st>>>(tab, n, total, chunk, (const R*)fld, partials);
         else
-            ens_sum_kernel<R, B, 1><<<grid, kBS, 0, st>>>(tab, n, total, (const R*)fld, partials);
+            ens_sum_kernel<R, B, 1><<<grid, kBS, 0, st>>>(tab, n, total, chunk, (const R*)fld, partials);
     });
     return hipGetLastError();
 }
