@@ -39,6 +39,13 @@ hipError_t launch_fmunu(const Lat& L, const void* U, int is_f32, int mu, int nu,
                         int out_layout, const int64_t strides[6], hipStream_t st);
 uint32_t magnetic_blocks(const Lat& L, int is_f32);
 int magnetic_default_variant(int is_f32);
+// rectangular Wilson-loop table (k_wloops.hip): R, T up to kMaxLoopRT
+constexpr int kMaxLoopRT = 6;
+uint32_t wloop_blocks(const Lat& L, int tslice);
+size_t wloop_partial_bytes(const Lat& L, int tslice);
+hipError_t launch_wloop_table(const Lat& L, const void* U, int is_f32, int mu, int nu, int Rmax, int Tmax,
+                              int tslice, double* partials, double* out, double scale, hipStream_t st);
+
 // number of magnetic kernel variants (forms x occupancy targets) the autotuner chooses from
 constexpr int kMagneticVariants = 6;
 hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st, int variant,

@@ -177,6 +177,17 @@ class GaugeField:
         s = self.path_trace(path, tau)
         return s / (self.V // self.dims[0] if tau else self.V)
 
+    def wilson_loops(self, mu, nu, Rmax=6, Tmax=6, tau=0):
+        """Table W[R-1, T-1] of wmunu(mu, nu, R, T) (tau = 0) or wmunutau(tau, mu, nu, R, T) for
+        1 <= R <= Rmax, 1 <= T <= Tmax, computed in one pass (src/wloops.F90:567-791)."""
+        out = torch.empty(int(Rmax) * int(Tmax), dtype=torch.float64, device=self.device)
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_wilson_loops(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu), int(Rmax),
+                                               int(Tmax), int(tau), _ptr(out), _ptr(ws), ws.numel(), _stream()),
+                  "flue_wilson_loops")
+        return out.cpu().numpy().reshape(int(Rmax), int(Tmax))
+
     def generic_path(self, coordBase, path):
         """genericPath at one 1-based base point: the transposed ordered product (src/wloops.F90:20-57)."""
         out = torch.empty(18, dtype=torch.float64, device=self.device)

@@ -170,6 +170,14 @@ int flue_polyakov(const int32_t dims[4], const void* U, int32_t prec, double* d_
 int flue_path_trace(const int32_t dims[4], const void* U, int32_t prec, const int32_t* path, int32_t npath,
                     int32_t tau, double* d_sum, void* ws, size_t ws_bytes, void* stream);
 
+/* every rectangular Wilson loop of the plane (mu, nu) with 1 <= R <= Rmax steps along mu
+ * and 1 <= T <= Tmax along nu (Rmax, Tmax <= 6), in one pass:
+ * d_out[(R-1)*Tmax + (T-1)] = wmunu(data, mu, nu, R, T) for tau = 0 (src/wloops.F90:679-791)
+ * or wmunutau(data, tau, mu, nu, R, T) for the 1-based time slice tau (:567-677);
+ * mu != nu.  ws needs flue_workspace_bytes(dims) bytes. */
+int flue_wilson_loops(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, int32_t nu, int32_t Rmax,
+                      int32_t Tmax, int32_t tau, double* d_out, void* ws, size_t ws_bytes, void* stream);
+
 /* genericPath at one 1-based base point, transposed, as 9 complex128 in Fortran order
  * written to device memory d_out */
 int flue_generic_path(const int32_t dims[4], const void* U, int32_t prec, const int32_t coordBase[4],

@@ -524,6 +524,7 @@ int or_wmunu(const cplx* data, const int32_t dims[4], int32_t mu, int32_t nu, in
              double* loopVal) {
     if (check_rect(dims, mu, nu, nMu, nNu)) return ORC_EINVAL;
     double s = 0.0;
+#pragma omp parallel for collapse(3) reduction(+ : s) schedule(static)
     for (int x = 1; x <= dims[1]; ++x)
         for (int y = 1; y <= dims[2]; ++y)
             for (int z = 1; z <= dims[3]; ++z)
@@ -540,6 +541,7 @@ int or_wmunutau(const cplx* data, const int32_t dims[4], int32_t tau, int32_t mu
     if (check_rect(dims, mu, nu, nMu, nNu)) return ORC_EINVAL;
     if (tau < 1 || tau > dims[0]) return ORC_EINVAL;
     double s = 0.0;
+#pragma omp parallel for collapse(3) reduction(+ : s) schedule(static)
     for (int x = 1; x <= dims[1]; ++x)
         for (int y = 1; y <= dims[2]; ++y)
             for (int z = 1; z <= dims[3]; ++z) {

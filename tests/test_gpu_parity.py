@@ -163,6 +163,65 @@ def test_wilson_loops(FL, small, mu, nu, R, T):
     assert rel(fc.wmunutau_c(small, 2, mu, nu, R, T), O.wmunutau(small, 2, mu, nu, R, T)) < 1e-12
 
 
+@pytest.mark.parametrize("mu,nu", [(2, 1), (1, 3), (4, 2)])
+def test_wilson_loop_table_fp64(FL, small, mu, nu):
+    # R, T up to 6 on extents 4..7: loops wrap the lattice more than once
+    g = gfield(FL, small)
+    W = g.wilson_loops(mu, nu, 6, 6)
+    Wt = g.wilson_loops(mu, nu, 4, 5, tau=2)
+    for R in range(1, 7):
+        for T in range(1, 7):
+            assert abs(W[R - 1, T - 1] - O.wmunu(small, mu, nu, R, T)) < 1e-13, (R, T)
+            if R <= 4 and T <= 5:
+                assert abs(Wt[R - 1, T - 1] - O.wmunutau(small, 2, mu, nu, R, T)) < 1e-13, (R, T)
+
+
+def test_wilson_loop_table_fp32_and_errors(FL, small):
+    W = gfield(FL, small, "fp32").wilson_loops(3, 2, 3, 6)
+    for R in range(1, 4):
+        for T in range(1, 7):
+            assert abs(W[R - 1, T - 1] - O.wmunu(small, 3, 2, R, T)) < 1e-6
+    g = gfield(FL, small)
+    for bad in ((2, 2, 6, 6), (2, 1, 7, 6), (2, 1, 6, 0), (0, 1, 2, 2)):
+        with pytest.raises(FL.FlueError):
+            g.wilson_loops(*bad)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    # 24^3 x 48, eps = 0.35 (the plain FP64 drop-in workload)
+    return O.random_field((48, 24, 24, 24), 1001, 0.35)
+
+
+def test_wilson_loop_table_24x48(FL, c1):
+    # R along each spatial direction, T along time, R, T in 1..6
+    g = gfield(FL, c1)
+    for mu in (2, 3, 4):
+        W = g.wilson_loops(mu, 1, 6, 6)
+        for R, T in ((1, 1), (2, 3), (6, 6), (5, 2)):
+            assert rel(W[R - 1, T - 1], O.wmunu(c1, mu, 1, R, T)) < 1e-12, (mu, R, T)
+        # every entry against the loop walked on its own
+        for R in range(1, 7):
+            for T in range(1, 7):
+                assert rel(W[R - 1, T - 1], g.wilson_loop(mu, 1, R, T)) < 1e-12, (mu, R, T)
+
+
+def test_loop5_sampled_sites_24x48(FL, c1):
+    # per-site F on all six planes at 64 sampled sites (Fortran site offsets)
+    NT, NX, NY, NZ = c1.shape[:4]
+    rng = np.random.default_rng(11)
+    sites = rng.choice(NT * NX * NY * NZ, size=64, replace=False)
+    t, r = sites % NT, sites // NT
+    x, r = r % NX, r // NX
+    y, z = r % NY, r // NY
+    g = gfield(FL, c1)
+    for mu in range(1, 5):
+        for nu in range(mu + 1, 5):
+            want = O.loop5_munu_sites(c1, mu, nu, sites)
+            got = g.fmunu(mu, nu)[t, x, y, z].cpu().numpy()
+            assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), (mu, nu)
+
+
 # ------------------------------------------------------------------ F_munu and B
 
 ALL_PLANES = [(mu, nu) for mu in range(1, 5) for nu in range(1, 5) if mu != nu]
