@@ -167,10 +167,22 @@ __device__ __forceinline__ Mat<typename FV::real> ext(const Quad<FV>& q, const M
 #define FLUE_SCHED_FENCE() ((void)0)
 #endif
 
+// Lines of length 2 and 3 from x along one direction, handed from a quadrant (where the
+// direction is d2) to the next one (where it is d1); both build them with the same
+// operations, so the hand-over leaves every result bit-identical.
+template <typename R>
+struct Lines {
+    Mat<R> l2, l3;
+};
+
 // the seven rectangles (1,1),(1,2),(1,3),(2,1),(2,2),(3,1),(3,3), a along d1 and b along
-// d2, or only the 1x1 leaf when CLOVER is set (cloverLoopMuNuCoord, src/wloops.F90:129-199)
-template <int S1, int S2, bool CLOVER, typename FV>
-__device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real>& h) {
+// d2, or only the 1x1 leaf when CLOVER is set (cloverLoopMuNuCoord, src/wloops.F90:129-199).
+// With IN the d1 lines of length 2 and 3 come from `in`; with OUT the d2 lines are saved
+// to `out` for the next quadrant.
+template <int S1, int S2, bool CLOVER, bool IN = false, bool OUT = false, typename FV>
+__device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real>& h,
+                                         const Lines<typename FV::real>* in = nullptr,
+                                         Lines<typename FV::real>* out = nullptr) {
     using R = typename FV::real;
     const R w11 = CLOVER ? R(1) : R(kK1);
     const R w22 = R(kK2), w33 = R(kK5), w12 = R(0.5 * kK3), w13 = R(0.5 * kK4);
@@ -184,16 +196,22 @@ __device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real
         FLUE_SCHED_FENCE();
         CA = ext<S2, 2, 1, 1>(q, CA);               // CA(1,2)
         Ld = ext<S2, 2, 0, 1>(q, Ld);               // Line(x,d2,2)
+        if constexpr (OUT) out->l2 = Ld;
         Mat<R> CB2 = ext<S1, 1, 0, 2>(q, Ld);       // CB(2,1)
         acc_ah(h, w12, CA, CB2);                    // W(1,2)
         FLUE_SCHED_FENCE();
         CA = ext<S2, 2, 1, 2>(q, CA);               // CA(1,3)
         Ld = ext<S2, 2, 0, 2>(q, Ld);               // Line(x,d2,3)
+        if constexpr (OUT) out->l3 = Ld;
         Mat<R> CB3 = ext<S1, 1, 0, 3>(q, Ld);       // CB(3,1)
         acc_ah(h, w13, CA, CB3);                    // W(1,3)
         FLUE_SCHED_FENCE();
         // row a = 2
-        Mat<R> L = ext<S1, 1, 1, 0>(q, L1);         // Line(x,d1,2)
+        Mat<R> L;
+        if constexpr (IN)
+            L = in->l2;                             // Line(x,d1,2)
+        else
+            L = ext<S1, 1, 1, 0>(q, L1);
         CA = ext<S2, 2, 2, 0>(q, L);                // CA(2,1)
         CB1 = ext<S1, 1, 1, 1>(q, CB1);             // CB(1,2)
         acc_ah(h, w12, CA, CB1);                    // W(2,1)
@@ -203,7 +221,10 @@ __device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real
         acc_ah(h, w22, CA, CB2);                    // W(2,2)
         FLUE_SCHED_FENCE();
         // row a = 3
-        L = ext<S1, 1, 2, 0>(q, L);                 // Line(x,d1,3)
+        if constexpr (IN)
+            L = in->l3;                             // Line(x,d1,3)
+        else
+            L = ext<S1, 1, 2, 0>(q, L);
         CA = ext<S2, 2, 3, 0>(q, L);                // CA(3,1)
         CB1 = ext<S1, 1, 2, 1>(q, CB1);             // CB(1,3)
         acc_ah(h, w13, CA, CB1);                    // W(3,1)
@@ -217,15 +238,25 @@ __device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real
     FLUE_SCHED_FENCE();
 }
 
-// the four quadrants in counter-clockwise order: (+mu,+nu), (+nu,-mu), (-mu,-nu), (-nu,+mu)
-template <bool CLOVER, typename FV>
+// the four quadrants in counter-clockwise order: (+mu,+nu), (+nu,-mu), (-mu,-nu), (-nu,+mu).
+// Each quadrant's d2 is the next one's d1; with SHARE the lines along it are computed once.
+template <bool CLOVER, bool SHARE = false, typename FV>
 __device__ __forceinline__ AH<typename FV::real> plane_ah(const PlaneGeom<FV>& g) {
-    AH<typename FV::real> h;
+    using R = typename FV::real;
+    AH<R> h;
     ah_zero(h);
-    quadrant<1, 1, CLOVER>(make_quad<1, 1, true>(g), h);
-    quadrant<1, -1, CLOVER>(make_quad<1, -1, false>(g), h);
-    quadrant<-1, -1, CLOVER>(make_quad<-1, -1, true>(g), h);
-    quadrant<-1, 1, CLOVER>(make_quad<-1, 1, false>(g), h);
+    if constexpr (SHARE && !CLOVER) {
+        Lines<R> a, b;
+        quadrant<1, 1, false, false, true>(make_quad<1, 1, true>(g), h, nullptr, &a);
+        quadrant<1, -1, false, true, true>(make_quad<1, -1, false>(g), h, &a, &b);
+        quadrant<-1, -1, false, true, true>(make_quad<-1, -1, true>(g), h, &b, &a);
+        quadrant<-1, 1, false, true, false>(make_quad<-1, 1, false>(g), h, &a, nullptr);
+    } else {
+        quadrant<1, 1, CLOVER>(make_quad<1, 1, true>(g), h);
+        quadrant<1, -1, CLOVER>(make_quad<1, -1, false>(g), h);
+        quadrant<-1, -1, CLOVER>(make_quad<-1, -1, true>(g), h);
+        quadrant<-1, 1, CLOVER>(make_quad<-1, 1, false>(g), h);
+    }
     return h;
 }
 
