@@ -142,8 +142,8 @@ __device__ __forceinline__ Mat<typename FV::real> first_link(const Quad<FV>& q) 
     }
 }
 
-// M * (link along d1 or d2 leaving P(a1, a2))
-template <int S, int AX, int a1, int a2, typename FV>
+// M * (link along d1 or d2 leaving P(a1, a2)); with C2 only rows 0 and 1 are formed
+template <int S, int AX, int a1, int a2, bool C2 = false, typename FV>
 __device__ __forceinline__ Mat<typename FV::real> ext(const Quad<FV>& q, const Mat<typename FV::real>& M) {
     using R = typename FV::real;
     const int ax = AX == 1 ? q.ax1 : q.ax2;
@@ -151,9 +151,22 @@ __device__ __forceinline__ Mat<typename FV::real> ext(const Quad<FV>& q, const M
     constexpr int b2 = (AX == 2 && S < 0) ? a2 + 1 : a2;
     const Mat<R> U = load_link(q.f, ax, q.sp + q.T1[b1] + q.T2[b2]);
     if constexpr (S > 0)
-        return mul(M, U);
+        return C2 ? mul2(M, U) : mul(M, U);
     else
-        return mul_nd(M, U);
+        return C2 ? mul2_nd(M, U) : mul_nd(M, U);
+}
+
+// acc_ah on chain ends; with C2 the chains carry rows 0 and 1 and row 2 is rebuilt here
+template <bool C2, typename R>
+__device__ __forceinline__ void acc_leaf(AH<R>& h, R w, const Mat<R>& CA, const Mat<R>& CB) {
+    if constexpr (C2) {
+        Mat<R> A = CA, B = CB;
+        fill_row2(A);
+        fill_row2(B);
+        acc_ah(h, w, A, B);
+    } else {
+        acc_ah(h, w, CA, CB);
+    }
 }
 
 // Scheduling fences keep each group's link loads next to the products that use them;
@@ -179,7 +192,7 @@ struct Lines {
 // d2, or only the 1x1 leaf when CLOVER is set (cloverLoopMuNuCoord, src/wloops.F90:129-199).
 // With IN the d1 lines of length 2 and 3 come from `in`; with OUT the d2 lines are saved
 // to `out` for the next quadrant.
-template <int S1, int S2, bool CLOVER, bool IN = false, bool OUT = false, typename FV>
+template <int S1, int S2, bool CLOVER, bool IN = false, bool OUT = false, bool C2 = false, typename FV>
 __device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real>& h,
                                          const Lines<typename FV::real>* in = nullptr,
                                          Lines<typename FV::real>* out = nullptr) {
@@ -188,74 +201,75 @@ __device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real
     const R w22 = R(kK2), w33 = R(kK5), w12 = R(0.5 * kK3), w13 = R(0.5 * kK4);
     // row a = 1
     const Mat<R> L1 = first_link<S1, 1, 0, 0>(q);   // Line(x,d1,1)
-    Mat<R> CA = ext<S2, 2, 1, 0>(q, L1);            // CA(1,1)
+    Mat<R> CA = ext<S2, 2, 1, 0, C2>(q, L1);            // CA(1,1)
     Mat<R> Ld = first_link<S2, 2, 0, 0>(q);         // Line(x,d2,1)
-    Mat<R> CB1 = ext<S1, 1, 0, 1>(q, Ld);           // CB(1,1)
-    acc_ah(h, w11, CA, CB1);                        // W(1,1)
+    Mat<R> CB1 = ext<S1, 1, 0, 1, C2>(q, Ld);           // CB(1,1)
+    acc_leaf<C2>(h, w11, CA, CB1);                        // W(1,1)
     if constexpr (!CLOVER) {
         FLUE_SCHED_FENCE();
-        CA = ext<S2, 2, 1, 1>(q, CA);               // CA(1,2)
-        Ld = ext<S2, 2, 0, 1>(q, Ld);               // Line(x,d2,2)
+        CA = ext<S2, 2, 1, 1, C2>(q, CA);               // CA(1,2)
+        Ld = ext<S2, 2, 0, 1, C2>(q, Ld);               // Line(x,d2,2)
         if constexpr (OUT) out->l2 = Ld;
-        Mat<R> CB2 = ext<S1, 1, 0, 2>(q, Ld);       // CB(2,1)
-        acc_ah(h, w12, CA, CB2);                    // W(1,2)
+        Mat<R> CB2 = ext<S1, 1, 0, 2, C2>(q, Ld);       // CB(2,1)
+        acc_leaf<C2>(h, w12, CA, CB2);                    // W(1,2)
         FLUE_SCHED_FENCE();
-        CA = ext<S2, 2, 1, 2>(q, CA);               // CA(1,3)
-        Ld = ext<S2, 2, 0, 2>(q, Ld);               // Line(x,d2,3)
+        CA = ext<S2, 2, 1, 2, C2>(q, CA);               // CA(1,3)
+        Ld = ext<S2, 2, 0, 2, C2>(q, Ld);               // Line(x,d2,3)
         if constexpr (OUT) out->l3 = Ld;
-        Mat<R> CB3 = ext<S1, 1, 0, 3>(q, Ld);       // CB(3,1)
-        acc_ah(h, w13, CA, CB3);                    // W(1,3)
+        Mat<R> CB3 = ext<S1, 1, 0, 3, C2>(q, Ld);       // CB(3,1)
+        acc_leaf<C2>(h, w13, CA, CB3);                    // W(1,3)
         FLUE_SCHED_FENCE();
         // row a = 2
         Mat<R> L;
         if constexpr (IN)
             L = in->l2;                             // Line(x,d1,2)
         else
-            L = ext<S1, 1, 1, 0>(q, L1);
-        CA = ext<S2, 2, 2, 0>(q, L);                // CA(2,1)
-        CB1 = ext<S1, 1, 1, 1>(q, CB1);             // CB(1,2)
-        acc_ah(h, w12, CA, CB1);                    // W(2,1)
+            L = ext<S1, 1, 1, 0, C2>(q, L1);
+        CA = ext<S2, 2, 2, 0, C2>(q, L);                // CA(2,1)
+        CB1 = ext<S1, 1, 1, 1, C2>(q, CB1);             // CB(1,2)
+        acc_leaf<C2>(h, w12, CA, CB1);                    // W(2,1)
         FLUE_SCHED_FENCE();
-        CA = ext<S2, 2, 2, 1>(q, CA);               // CA(2,2)
-        CB2 = ext<S1, 1, 1, 2>(q, CB2);             // CB(2,2)
-        acc_ah(h, w22, CA, CB2);                    // W(2,2)
+        CA = ext<S2, 2, 2, 1, C2>(q, CA);               // CA(2,2)
+        CB2 = ext<S1, 1, 1, 2, C2>(q, CB2);             // CB(2,2)
+        acc_leaf<C2>(h, w22, CA, CB2);                    // W(2,2)
         FLUE_SCHED_FENCE();
         // row a = 3
         if constexpr (IN)
             L = in->l3;                             // Line(x,d1,3)
         else
-            L = ext<S1, 1, 2, 0>(q, L);
-        CA = ext<S2, 2, 3, 0>(q, L);                // CA(3,1)
-        CB1 = ext<S1, 1, 2, 1>(q, CB1);             // CB(1,3)
-        acc_ah(h, w13, CA, CB1);                    // W(3,1)
+            L = ext<S1, 1, 2, 0, C2>(q, L);
+        CA = ext<S2, 2, 3, 0, C2>(q, L);                // CA(3,1)
+        CB1 = ext<S1, 1, 2, 1, C2>(q, CB1);             // CB(1,3)
+        acc_leaf<C2>(h, w13, CA, CB1);                    // W(3,1)
         FLUE_SCHED_FENCE();
-        CA = ext<S2, 2, 3, 1>(q, CA);               // CA(3,2)
-        CA = ext<S2, 2, 3, 2>(q, CA);               // CA(3,3)
-        CB3 = ext<S1, 1, 1, 3>(q, CB3);             // CB(3,2)
-        CB3 = ext<S1, 1, 2, 3>(q, CB3);             // CB(3,3)
-        acc_ah(h, w33, CA, CB3);                    // W(3,3)
+        CA = ext<S2, 2, 3, 1, C2>(q, CA);               // CA(3,2)
+        CA = ext<S2, 2, 3, 2, C2>(q, CA);               // CA(3,3)
+        CB3 = ext<S1, 1, 1, 3, C2>(q, CB3);             // CB(3,2)
+        CB3 = ext<S1, 1, 2, 3, C2>(q, CB3);             // CB(3,3)
+        acc_leaf<C2>(h, w33, CA, CB3);                    // W(3,3)
     }
     FLUE_SCHED_FENCE();
 }
 
 // the four quadrants in counter-clockwise order: (+mu,+nu), (+nu,-mu), (-mu,-nu), (-nu,+mu).
 // Each quadrant's d2 is the next one's d1; with SHARE the lines along it are computed once.
-template <bool CLOVER, bool SHARE = false, typename FV>
+// With C2 the link chains carry two rows (SU(3) links only; see mul2).
+template <bool CLOVER, bool SHARE = false, bool C2 = false, typename FV>
 __device__ __forceinline__ AH<typename FV::real> plane_ah(const PlaneGeom<FV>& g) {
     using R = typename FV::real;
     AH<R> h;
     ah_zero(h);
     if constexpr (SHARE && !CLOVER) {
         Lines<R> a, b;
-        quadrant<1, 1, false, false, true>(make_quad<1, 1, true>(g), h, nullptr, &a);
-        quadrant<1, -1, false, true, true>(make_quad<1, -1, false>(g), h, &a, &b);
-        quadrant<-1, -1, false, true, true>(make_quad<-1, -1, true>(g), h, &b, &a);
-        quadrant<-1, 1, false, true, false>(make_quad<-1, 1, false>(g), h, &a, nullptr);
+        quadrant<1, 1, false, false, true, C2>(make_quad<1, 1, true>(g), h, nullptr, &a);
+        quadrant<1, -1, false, true, true, C2>(make_quad<1, -1, false>(g), h, &a, &b);
+        quadrant<-1, -1, false, true, true, C2>(make_quad<-1, -1, true>(g), h, &b, &a);
+        quadrant<-1, 1, false, true, false, C2>(make_quad<-1, 1, false>(g), h, &a, nullptr);
     } else {
-        quadrant<1, 1, CLOVER>(make_quad<1, 1, true>(g), h);
-        quadrant<1, -1, CLOVER>(make_quad<1, -1, false>(g), h);
-        quadrant<-1, -1, CLOVER>(make_quad<-1, -1, true>(g), h);
-        quadrant<-1, 1, CLOVER>(make_quad<-1, 1, false>(g), h);
+        quadrant<1, 1, CLOVER, false, false, C2>(make_quad<1, 1, true>(g), h);
+        quadrant<1, -1, CLOVER, false, false, C2>(make_quad<1, -1, false>(g), h);
+        quadrant<-1, -1, CLOVER, false, false, C2>(make_quad<-1, -1, true>(g), h);
+        quadrant<-1, 1, CLOVER, false, false, C2>(make_quad<-1, 1, false>(g), h);
     }
     return h;
 }

@@ -220,6 +220,72 @@ __device__ __forceinline__ Mat<R> mul_nd(const Mat<R>& A, const Mat<R>& B) {
     return C;
 }
 
+// Rows 0 and 1 of A * B and of A * B^dagger (row 2 of the result is left zero).  For
+// SU(3) matrices row 2 follows from the other two (fill_row2), so a product chain of
+// SU(3) links can carry two rows: 72 instead of 108 multiply-adds per step.
+template <typename R>
+__device__ __forceinline__ Mat<R> mul2(const Mat<R>& A, const Mat<R>& B) {
+    Mat<R> C;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            R cr = A.r[a * 3] * B.r[b];
+            R ci = A.r[a * 3] * B.i[b];
+            cr = fmad(-A.i[a * 3], B.i[b], cr);
+            ci = fmad(A.i[a * 3], B.r[b], ci);
+#pragma unroll
+            for (int k = 1; k < 3; ++k) cmac(cr, ci, A.r[a * 3 + k], A.i[a * 3 + k], B.r[k * 3 + b], B.i[k * 3 + b]);
+            C.r[a * 3 + b] = cr;
+            C.i[a * 3 + b] = ci;
+        }
+#pragma unroll
+    for (int b = 6; b < 9; ++b) C.r[b] = C.i[b] = R(0);
+    return C;
+}
+
+template <typename R>
+__device__ __forceinline__ Mat<R> mul2_nd(const Mat<R>& A, const Mat<R>& B) {
+    Mat<R> C;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            R cr = A.r[a * 3] * B.r[b * 3];
+            R ci = A.i[a * 3] * B.r[b * 3];
+            cr = fmad(A.i[a * 3], B.i[b * 3], cr);
+            ci = fmad(-A.r[a * 3], B.i[b * 3], ci);
+#pragma unroll
+            for (int k = 1; k < 3; ++k)
+                cmac_c(cr, ci, A.r[a * 3 + k], A.i[a * 3 + k], B.r[b * 3 + k], B.i[b * 3 + k]);
+            C.r[a * 3 + b] = cr;
+            C.i[a * 3 + b] = ci;
+        }
+#pragma unroll
+    for (int b = 6; b < 9; ++b) C.r[b] = C.i[b] = R(0);
+    return C;
+}
+
+// row 2 = conj(row 0 x row 1), exact for SU(3) (unitary, det 1)
+template <typename R>
+__device__ __forceinline__ void fill_row2(Mat<R>& M) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int j = (c + 1) % 3, k = (c + 2) % 3;
+        // x = r0_j r1_k - r0_k r1_j; row2_c = conj(x)
+        R xr = M.r[j] * M.r[3 + k];
+        R xi = M.r[j] * M.i[3 + k];
+        xr = fmad(-M.i[j], M.i[3 + k], xr);
+        xi = fmad(M.i[j], M.r[3 + k], xi);
+        xr = fmad(-M.r[k], M.r[3 + j], xr);
+        xr = fmad(M.i[k], M.i[3 + j], xr);
+        xi = fmad(-M.r[k], M.i[3 + j], xi);
+        xi = fmad(-M.i[k], M.r[3 + j], xi);
+        M.r[6 + c] = xr;
+        M.i[6 + c] = -xi;
+    }
+}
+
 // C = A^dagger * B
 template <typename R>
 __device__ __forceinline__ Mat<R> mul_dn(const Mat<R>& A, const Mat<R>& B) {

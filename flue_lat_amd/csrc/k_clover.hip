@@ -63,9 +63,11 @@ __device__ __forceinline__ int plane_nu(int pl) { return pl == 0 ? 1 : (pl == 1 
 template <typename FV, int WPE, bool PLANE_MAJOR>
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ partials) {
-    // consecutive quadrants share their lines where the registers allow it (FP32 at up to
-    // two waves per SIMD); at three waves, or in FP64, the extra live lines would spill
-    constexpr bool kShare = sizeof(typename FV::real) == 4 && WPE <= 2;
+    // FP32: link chains carry two rows (SU(3) links; row 2 is rebuilt where a leaf closes),
+    // and consecutive quadrants share their lines where the registers allow it (up to two
+    // waves per SIMD).  FP64 keeps full products, so it makes no assumption on the links.
+    constexpr bool kC2 = sizeof(typename FV::real) == 4;
+    constexpr bool kShare = kC2 && WPE <= 2;
     // the plane comes from blockIdx.y so that it, and every buffer descriptor derived
     // from it, stays in scalar registers (a division of blockIdx.x would not)
     const int pl = (int)blockIdx.y;
@@ -77,13 +79,13 @@ magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ pa
         if constexpr (PLANE_MAJOR) {
             const uint32_t s = plane_site(L, po, pl, o, c);
             const int mu = plane_mu(pl), nu = plane_nu(pl);
-            acc = f_norm2(plane_ah<false, kShare>(plane_geom(L, f, s, c[mu], c[nu], mu, nu)));
+            acc = f_norm2(plane_ah<false, kShare, kC2>(plane_geom(L, f, s, c[mu], c[nu], mu, nu)));
         } else {
             site_coords(L, o, c);
 #pragma unroll 1
             for (int p = 0; p < 3; ++p) {
                 const int mu = plane_mu(p), nu = plane_nu(p);
-                acc += f_norm2(plane_ah<false, kShare>(plane_geom(L, f, o, c[mu], c[nu], mu, nu)));
+                acc += f_norm2(plane_ah<false, kShare, kC2>(plane_geom(L, f, o, c[mu], c[nu], mu, nu)));
             }
         }
     }
@@ -110,7 +112,7 @@ fmunu_kernel(Lat L, FV f, int mu, int nu, void* __restrict__ out, int layout, Ou
     if (s >= L.V) return;
     int c[4];
     site_coords(L, s, c);
-    const AH<R> h = plane_ah<CLOVER, sizeof(R) == 4>(plane_geom(L, f, s, c[mu], c[nu], mu, nu));
+    const AH<R> h = plane_ah<CLOVER, sizeof(R) == 4, sizeof(R) == 4>(plane_geom(L, f, s, c[mu], c[nu], mu, nu));
     // F = A^T / 4: F_aa = i d_a / 4, F_ab = -conj(o_ab)/8, F_ba = o_ab/8 (a < b)
     R fr[9], fi[9];
     constexpr int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};

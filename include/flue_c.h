@@ -58,7 +58,11 @@ enum {
     FLUE_EWS = 10      /* workspace too small */
 };
 
-enum { FLUE_F64 = 0, FLUE_F32 = 1 };        /* link precision of a SoA field */
+/* link precision of a SoA field.  FLUE_F64 makes no assumption on the links.  FLUE_F32
+ * fields must hold SU(3) links (unitary, det 1, to FP32 rounding): the clover kernels
+ * carry two rows of each link product and rebuild the third as conj(row0 x row1).
+ * Sums and per-site traces are FP64 in both. */
+enum { FLUE_F64 = 0, FLUE_F32 = 1 };
 enum { FLUE_C128 = 0, FLUE_C64 = 1 };       /* element type of a complex array */
 enum { FLUE_OUT_STRIDED = 0, FLUE_OUT_SOA = 1 };
 
