@@ -105,9 +105,11 @@ def main():
     torch.cuda.synchronize()
 
     res = {}
+    tuned = {}
     for name, g in (("fp32", f32), ("fp64", f64)):
-        out = torch.empty(1, dtype=torch.float64, device="cuda")
-        g.workspace()
+        # the autotuner times the kernel variants on this field (outside the timed region);
+        # g.magnetic() then runs the fastest
+        tuned[name] = g.autotune("magnetic", reps=3)[0]
         holder = {}
 
         def step(g=g, holder=holder):
@@ -116,7 +118,6 @@ def main():
         t_ev, t_wall = time_steps(step, a.steps, a.warmup, torch)
         B = holder["B"].item()
         res[name] = {"s_per_step": t_ev, "wall_s_per_step": t_wall, "B": B, "gsites": V / t_ev / 1e9}
-        del out
 
     # companion observables of the same configuration (plaquette, Polyakov), FP32 vs FP64
     extras = {}
@@ -148,6 +149,7 @@ def main():
                    "lattice": "48^3x96", "sites": V, "eps": EPS, "seed": SEED},
         "fp64": {"value": round(res["fp64"]["gsites"], 6), "ms_per_step": round(res["fp64"]["s_per_step"] * 1e3, 6)},
         "fp32_over_fp64": round(res["fp32"]["gsites"] / res["fp64"]["gsites"], 4),
+        "kernel_variant": tuned,
         "rel_err_B_fp32_vs_fp64": rel_B,
         "B": {"fp32": res["fp32"]["B"], "fp64": res["fp64"]["B"]},
         "observables": extras,

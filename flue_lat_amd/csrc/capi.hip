@@ -81,7 +81,10 @@ int get_ws(void* ws, size_t ws_bytes, size_t need, void** out) {
     return FLUE_OK;
 }
 
-size_t partial_bytes(uint32_t nblocks) { return (size_t)nblocks * 2 * sizeof(double) + 256; }
+// per-block compensated partials plus the tail area of the two-stage finalize
+size_t partial_bytes(uint32_t nblocks) {
+    return ((size_t)nblocks + (nblocks + kFinalChunk - 1) / kFinalChunk) * 2 * sizeof(double) + 256;
+}
 
 // ---------------------------------------------------------------- autotuner table
 // (device, op, precision, NT, NX, NY, NZ) -> fastest kernel variant measured on it

@@ -46,6 +46,9 @@ size_t wloop_partial_bytes(const Lat& L, int tslice);
 hipError_t launch_wloop_table(const Lat& L, const void* U, int is_f32, int mu, int nu, int Rmax, int Tmax,
                               int tslice, double* partials, double* out, double scale, hipStream_t st);
 
+// partial sums per block of the first stage of a long fixed-order reduction
+constexpr uint32_t kFinalChunk = 2048;
+
 // number of magnetic kernel variants (forms x occupancy targets) the autotuner chooses from
 constexpr int kMagneticVariants = 6;
 hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st, int variant,

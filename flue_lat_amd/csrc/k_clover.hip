@@ -63,11 +63,12 @@ __device__ __forceinline__ int plane_nu(int pl) { return pl == 0 ? 1 : (pl == 1 
 template <typename FV, int WPE, bool PLANE_MAJOR>
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ partials) {
-    // FP32: link chains carry two rows (SU(3) links; row 2 is rebuilt where a leaf closes),
-    // and consecutive quadrants share their lines where the registers allow it (up to two
-    // waves per SIMD).  FP64 keeps full products, so it makes no assumption on the links.
+    // FP32: link chains carry two rows (SU(3) links; row 2 is rebuilt where a leaf closes).
+    // FP64 keeps full products, so it makes no assumption on the links.  Consecutive
+    // quadrants share their lines where the registers allow it: FP32 at up to two waves
+    // per SIMD, FP64 at one.
     constexpr bool kC2 = sizeof(typename FV::real) == 4;
-    constexpr bool kShare = kC2 && WPE <= 2;
+    constexpr bool kShare = kC2 ? WPE <= 2 : WPE == 1;
     // the plane comes from blockIdx.y so that it, and every buffer descriptor derived
     // from it, stays in scalar registers (a division of blockIdx.x would not)
     const int pl = (int)blockIdx.y;
@@ -162,7 +163,7 @@ static PlaneOrder plane_order(const Lat& L, size_t real_bytes) {
     return po;
 }
 
-int magnetic_default_variant(int is_f32) { return is_f32 ? 2 : 4; }
+int magnetic_default_variant(int is_f32) { return is_f32 ? 5 : 4; }
 
 // variant 1..3: site-major with an occupancy target of 1..3 waves per SIMD;
 // variant 4..6: plane-major with 1..3 waves per SIMD; 0 picks the precision's default.

@@ -164,6 +164,20 @@ __global__ void __launch_bounds__(kBS) finalize_kernel(const double* __restrict_
     if (threadIdx.x == 0) out[0] = offset + (r.hi + r.lo) * scale;
 }
 
+// stage one of a long reduction: block b sums entries [b*kFinalChunk, (b+1)*kFinalChunk)
+__global__ void __launch_bounds__(kBS) finalize_stage_kernel(const double* __restrict__ partials, uint32_t n,
+                                                             double* __restrict__ out) {
+    const uint32_t lo = blockIdx.x * kFinalChunk;
+    const uint32_t hi = lo + kFinalChunk < n ? lo + kFinalChunk : n;
+    DD acc = {0.0, 0.0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBS) acc = dd_add(acc, DD{partials[2 * i], partials[2 * i + 1]});
+    const DD r = block_sum<kBS>(acc);
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = r.hi;
+        out[2 * blockIdx.x + 1] = r.lo;
+    }
+}
+
 PathArg make_path(const int32_t* path, int npath) {
     PathArg p;
     p.n = npath;
@@ -214,15 +228,25 @@ hipError_t launch_generic_path(const Lat& L, const void* U, int is_f32, const in
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const double* partials, uint32_t n, double* out, double scale, hipStream_t st) {
-    finalize_kernel<<<1, kBS, 0, st>>>(partials, n, out, scale, 0.0);
+// Long partial lists are reduced in two fixed-order stages: blocks of kFinalChunk
+// entries into the tail area behind the list (partial_bytes reserves it), then those.
+hipError_t launch_finalize_affine(const double* partials, uint32_t n, double* out, double scale, double offset,
+                                  hipStream_t st) {
+    if (n <= 2 * kFinalChunk) {
+        finalize_kernel<<<1, kBS, 0, st>>>(partials, n, out, scale, offset);
+        return hipGetLastError();
+    }
+    const uint32_t n2 = (n + kFinalChunk - 1) / kFinalChunk;
+    double* tail = const_cast<double*>(partials) + 2 * (size_t)n;
+    finalize_stage_kernel<<<n2, kBS, 0, st>>>(partials, n, tail);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    finalize_kernel<<<1, kBS, 0, st>>>(tail, n2, out, scale, offset);
     return hipGetLastError();
 }
 
-hipError_t launch_finalize_affine(const double* partials, uint32_t n, double* out, double scale, double offset,
-                                  hipStream_t st) {
-    finalize_kernel<<<1, kBS, 0, st>>>(partials, n, out, scale, offset);
-    return hipGetLastError();
+hipError_t launch_finalize(const double* partials, uint32_t n, double* out, double scale, hipStream_t st) {
+    return launch_finalize_affine(partials, n, out, scale, 0.0, st);
 }
 
 }  // namespace flue
