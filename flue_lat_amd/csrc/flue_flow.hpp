@@ -132,16 +132,22 @@ __device__ __forceinline__ double plaquette_site(const Lat& L, const FV& f, uint
     using R = typename FV::real;
     int c[4];
     site_coords(L, s, c);
+    // forward neighbour of s along each direction
+    uint32_t sf[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) sf[d] = c[d] == L.n[d] - 1 ? s - (uint32_t)(L.n[d] - 1) * L.st[d] : s + L.st[d];
+    // the six planes unrolled (the enabled set is uniform), so the link loads of several
+    // planes are in flight together
     double acc = 0.0;
-    for (int mu = mu0; mu <= mu1; ++mu) {
+#pragma unroll
+    for (int mu = 0; mu < 3; ++mu) {
+        if (mu < mu0 || mu > mu1) continue;
         const Mat<R> Umu = load_link(f, mu, s);
-        int cm[4] = {c[0], c[1], c[2], c[3]};
-        const uint32_t sm = step_fwd(L, s, cm, mu);
-        for (int nu = mu + 1; nu <= nu1; ++nu) {
-            int cn[4] = {c[0], c[1], c[2], c[3]};
-            const uint32_t sn = step_fwd(L, s, cn, nu);
-            const Mat<R> A = mul(Umu, load_link(f, nu, sm));
-            const Mat<R> B = mul(load_link(f, nu, s), load_link(f, mu, sn));
+#pragma unroll
+        for (int nu = mu + 1; nu < 4; ++nu) {
+            if (nu > nu1) continue;
+            const Mat<R> A = mul(Umu, load_link(f, nu, sf[mu]));
+            const Mat<R> B = mul(load_link(f, nu, s), load_link(f, mu, sf[nu]));
             acc += (double)retr_nd(A, B);
         }
     }

@@ -1,5 +1,6 @@
 // k_loops.hip -- plaquette family, Polyakov loop, generic-path products and the
 // deterministic second pass of every lattice sum.
+#include "flue_flow.hpp"
 #include "flue_launch.hpp"
 
 namespace flue {
@@ -30,26 +31,10 @@ __device__ __forceinline__ void write_partial(double* partials, uint32_t blk, DD
 template <typename FV>
 __global__ void __launch_bounds__(kBS) plaquette_kernel(Lat L, FV f, int mu0, int mu1, int nu1,
                                                         double* __restrict__ partials) {
-    using R = typename FV::real;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kBS + threadIdx.x;
     double acc = 0.0;
-    if (s < L.V) {
-        int c[4];
-        site_coords(L, s, c);
-        for (int mu = mu0; mu <= mu1; ++mu) {
-            const Mat<R> Umu = load_link(f, mu, s);
-            int cm[4] = {c[0], c[1], c[2], c[3]};
-            const uint32_t sm = step_fwd(L, s, cm, mu);
-            for (int nu = mu + 1; nu <= nu1; ++nu) {
-                int cn[4] = {c[0], c[1], c[2], c[3]};
-                const uint32_t sn = step_fwd(L, s, cn, nu);
-                const Mat<R> A = mul(Umu, load_link(f, nu, sm));
-                const Mat<R> B = mul(load_link(f, nu, s), load_link(f, mu, sn));
-                acc += (double)retr_nd(A, B);
-            }
-        }
-    }
+    if (s < L.V) acc = plaquette_site(L, f, s, mu0, mu1, nu1);
     write_partial(partials, blk, block_sum<kBS>(DD{acc, 0.0}));
 }
 
