@@ -293,6 +293,8 @@ __device__ __forceinline__ double f_norm2(const AH<R>& h) {
 // (src/wloops.F90:502-506, 530-533)
 template <typename FV>
 __device__ __forceinline__ double magnetic_site(const Lat& L, const FV& f, uint32_t s) {
+    // FP32: two-row chains and shared lines, as in the single-field kernel
+    constexpr bool kC2 = sizeof(typename FV::real) == 4;
     int c[4];
     site_coords(L, s, c);
     double acc = 0.0;
@@ -302,7 +304,7 @@ __device__ __forceinline__ double magnetic_site(const Lat& L, const FV& f, uint3
         const int nu = pl == 0 ? 1 : (pl == 1 ? 3 : 2);
         const int cmu = pl == 0 ? c[2] : (pl == 1 ? c[1] : c[3]);
         const int cnu = pl == 0 ? c[1] : (pl == 1 ? c[3] : c[2]);
-        acc += f_norm2(plane_ah<false>(plane_geom(L, f, s, cmu, cnu, mu, nu)));
+        acc += f_norm2(plane_ah<false, kC2, kC2>(plane_geom(L, f, s, cmu, cnu, mu, nu)));
     }
     return acc;
 }

@@ -5,7 +5,7 @@
 
 namespace flue {
 
-// exp(X) by scaling and squaring a degree-16 Taylor polynomial (Horner form).
+// exp(X) by scaling and squaring a Taylor polynomial of degree <= 16 (Horner form).
 __device__ __forceinline__ Mat<double> su3_exp(const Mat<double>& X) {
     double nrm = 0.0;
 #pragma unroll
@@ -23,8 +23,16 @@ __device__ __forceinline__ Mat<double> su3_exp(const Mat<double>& X) {
         Y.r[e] = X.r[e] * scale;
         Y.i[e] = X.i[e] * scale;
     }
+    // Taylor degree: the smallest K <= 16 whose first dropped term, |Y|^(K+1)/(K+1)!, is
+    // below 1e-17 (|Y| <= 0.125 after the scaling, so K <= 11; small flow steps need ~8)
+    int K = 1;
+    double term = 0.5 * nrm * nrm;  // |Y|^2 / 2!, the first term a degree-1 polynomial drops
+    while (K < 16 && term > 1e-17) {
+        ++K;
+        term *= nrm / (double)(K + 1);
+    }
     Mat<double> E = identity<double>();
-    for (int k = 16; k >= 1; --k) {
+    for (int k = K; k >= 1; --k) {
         Mat<double> T = mul(Y, E);
         const double inv = 1.0 / (double)k;
 #pragma unroll

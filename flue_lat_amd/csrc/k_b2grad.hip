@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(kBS) magnetic_ah_kernel(Lat L, FV f, typename 
         const int nu = pl == 0 ? 1 : (pl == 1 ? 3 : 2);
         const int cmu = pl == 0 ? c[2] : (pl == 1 ? c[1] : c[3]);
         const int cnu = pl == 0 ? c[1] : (pl == 1 ? c[3] : c[2]);
-        const AH<R> h = plane_ah<false>(plane_geom(L, f, s, cmu, cnu, mu, nu));
+        const AH<R> h = plane_ah<false, sizeof(R) == 4, sizeof(R) == 4>(plane_geom(L, f, s, cmu, cnu, mu, nu));
         R* o = Abuf + (size_t)pl * 9 * L.V + s;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
