@@ -1,0 +1,71 @@
+"""Fields too large for the buffer-resource loads (18 V sizeof(real) >= 4 GiB per direction):
+the kernels then take the plain global-load view.  Checked through a size-independent
+property: a field tiled k times along t repeats every per-site value, so B and the
+Wilson-loop averages are unchanged, plaquette sums scale by k, and one flow step acts
+slice by slice.  Fields are 17 GB on the device.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+SMALL = (8, 16, 16, 16)
+VS = 16 ** 3
+
+
+def tiled(FL, g, k):
+    NT = g.dims[0]
+    data = g.data.view(72, NT, VS).repeat(1, k, 1).reshape(-1)
+    return FL.GaugeField((NT * k,) + g.dims[1:], g.precision, _data=data)
+
+
+@pytest.fixture(scope="module")
+def FL():
+    import flue_lat_amd as FL
+    return FL
+
+
+@pytest.mark.parametrize("prec,k", [("fp64", 920), ("fp32", 1830)])
+def test_unbuffered_view_matches_small_field(FL, prec, k):
+    bytes_per = 8 if prec == "fp64" else 4
+    assert 18 * SMALL[0] * k * VS * bytes_per >= 2 ** 32   # past the buffer-resource range
+    g = FL.GaugeField.random(SMALL, 77, 0.3, "fp64").astype(prec)
+    big = tiled(FL, g, k)
+    try:
+        B, Bb = g.magnetic(), big.magnetic()
+        assert abs(Bb - B) < 1e-12 * B, (B, Bb)
+        s, n = g.plaquette(1, 4, 4)
+        sb, nb = big.plaquette(1, 4, 4)
+        assert nb == k * n and abs(sb - k * s) < 1e-12 * abs(k * s)
+        W = g.wilson_loops(2, 1, 3, 6)
+        Wb = big.wilson_loops(2, 1, 3, 6)
+        assert np.abs(Wb - W).max() < 1e-12
+        # per-site F at sites of the last copy against the small field
+        F = g.fmunu(3, 1)
+        Fb = big.fmunu(3, 1)
+        off = (k - 1) * SMALL[0]
+        tol = (1e-13 if prec == "fp64" else 1e-6) * float(F.abs().max())
+        for t in (0, 3, 7):
+            assert float((Fb[off + t] - F[t]).abs().max()) <= tol
+        del F, Fb
+    finally:
+        del big
+        torch.cuda.empty_cache()
+
+
+def test_unbuffered_flow_step_acts_slice_by_slice(FL):
+    k = 920
+    g = FL.GaugeField.random(SMALL, 78, 0.3, "fp64")
+    big = tiled(FL, g, k)
+    try:
+        g.wilson_flow(0.02, 1, use_graph=False)
+        big.wilson_flow(0.02, 1, use_graph=False)
+        a = g.data.view(72, SMALL[0], VS)
+        b = big.data.view(72, SMALL[0] * k, VS)
+        for c in (0, k // 2, k - 1):
+            assert float((b[:, c * SMALL[0]:(c + 1) * SMALL[0]] - a).abs().max()) < 1e-13
+    finally:
+        del big
+        torch.cuda.empty_cache()
