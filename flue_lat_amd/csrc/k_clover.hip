@@ -65,10 +65,10 @@ __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(WPE, W
 magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ partials) {
     // FP32: link chains carry two rows (SU(3) links; row 2 is rebuilt where a leaf closes).
     // FP64 keeps full products, so it makes no assumption on the links.  Consecutive
-    // quadrants share their lines where the registers allow it: FP32 at up to two waves
-    // per SIMD, FP64 at one.
+    // quadrants share their lines where the registers allow it: always in FP32, at one
+    // wave per SIMD in FP64.
     constexpr bool kC2 = sizeof(typename FV::real) == 4;
-    constexpr bool kShare = kC2 ? WPE <= 2 : WPE == 1;
+    constexpr bool kShare = kC2 || WPE == 1;
     // the plane comes from blockIdx.y so that it, and every buffer descriptor derived
     // from it, stays in scalar registers (a division of blockIdx.x would not)
     const int pl = (int)blockIdx.y;
