@@ -604,7 +604,10 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
     hipStream_t st = S(stream);
     hipEvent_t e0, e1;
     FLUE_TRY(hipEventCreate(&e0));
-    FLUE_TRY(hipEventCreate(&e1));
+    if (hipError_t ec = hipEventCreate(&e1); ec != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_status(ec);
+    }
     int bv = 0;
     float bms = 0.f;
     hipError_t e = hipSuccess;
