@@ -52,10 +52,13 @@ struct Mat {
 
 // ---------------------------------------------------------------- field access
 
-// Read-only view of a SoA field.  BUF = true reads through one buffer resource per
-// direction (18 planes), so every load of a link shares one 32-bit vector offset and
-// takes its plane offset from a scalar register; it needs 18 * V * sizeof(R) < 2^32
-// bytes.  BUF = false is the plain global-memory form for larger fields.
+// Read-only view of a field: 36 complex planes (mu, a, b), each V sites of (re, im)
+// pairs, real index ((mu*9 + a*3 + b) * V + s) * 2 + ri.  A wave reads one plane of 64
+// consecutive sites with one 64-bit (FP32) or 128-bit (FP64) load per lane.  BUF = true
+// reads through one buffer resource per direction (9 planes), so every load of a link
+// shares one 32-bit vector offset and takes its plane offset from a scalar register; it
+// needs 18 * V * sizeof(R) < 2^32 bytes.  BUF = false is the plain global-memory form
+// for larger fields.
 template <typename R, bool BUF>
 struct FieldView {
     using real = R;
@@ -63,24 +66,37 @@ struct FieldView {
     const R* __restrict__ p;
     uint32_t V;
 
-    __device__ __forceinline__ const R* plane(int mu, int e, int ri) const {
-        return p + (size_t)(((mu * 9 + e) * 2) + ri) * V;
-    }
+    // complex plane (mu, e) with e = a*3 + b
+    __device__ __forceinline__ const R* cplane(int mu, int e) const { return p + (size_t)(mu * 9 + e) * 2 * V; }
     __device__ __forceinline__ __amdgpu_buffer_rsrc_t dir_rsrc(int mu) const {
         const char* b = (const char*)p + (size_t)mu * 18u * V * sizeof(R);
         // dword3 0x00020000: 32-bit data format, no swizzle -- the raw-buffer setting for gfx9 parts
         return __builtin_amdgcn_make_buffer_rsrc((void*)b, (short)0, (int)(18u * V * (uint32_t)sizeof(R)), 0x00020000);
     }
-    __device__ __forceinline__ R bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) const {
+    // one (re, im) pair: a 64-bit (FP32) or 128-bit (FP64) load, reinterpreted as a whole
+    // (taking dwords out of the loaded vector one by one is not used: with bit casts of
+    // single elements the compiler narrowed such loads to one dword)
+    __device__ __forceinline__ void bld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, R& re, R& im) const {
+        typedef R r2 __attribute__((ext_vector_type(2)));
+        r2 v;
         if constexpr (sizeof(R) == 4)
-            return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+            v = __builtin_bit_cast(r2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
         else
-            return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+            v = __builtin_bit_cast(r2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+        re = v.x;
+        im = v.y;
     }
-    // element (mu, e, ri) at site s
-    __device__ __forceinline__ R at(int mu, int e, int ri, uint32_t s) const {
-        return plane(mu, e, ri)[s];
-    }
+};
+
+template <typename R>
+struct Pair2;
+template <>
+struct Pair2<float> {
+    using type = float2;
+};
+template <>
+struct Pair2<double> {
+    using type = double2;
 };
 
 template <typename R, bool BUF>
@@ -88,17 +104,16 @@ __device__ __forceinline__ Mat<R> load_link(const FieldView<R, BUF>& f, int mu, 
     Mat<R> m;
     if constexpr (BUF) {
         const __amdgpu_buffer_rsrc_t r = f.dir_rsrc(mu);
-        const uint32_t vo = s * (uint32_t)sizeof(R), pb = f.V * (uint32_t)sizeof(R);
+        const uint32_t vo = s * 2u * (uint32_t)sizeof(R), pb = f.V * 2u * (uint32_t)sizeof(R);
 #pragma unroll
-        for (int e = 0; e < 9; ++e) {
-            m.r[e] = f.bld(r, vo, (2 * e) * pb);
-            m.i[e] = f.bld(r, vo, (2 * e + 1) * pb);
-        }
+        for (int e = 0; e < 9; ++e) f.bld2(r, vo, e * pb, m.r[e], m.i[e]);
     } else {
+        using P = typename Pair2<R>::type;
 #pragma unroll
         for (int e = 0; e < 9; ++e) {
-            m.r[e] = f.plane(mu, e, 0)[s];
-            m.i[e] = f.plane(mu, e, 1)[s];
+            const P v = reinterpret_cast<const P*>(f.cplane(mu, e))[s];
+            m.r[e] = v.x;
+            m.i[e] = v.y;
         }
     }
     return m;
@@ -142,10 +157,13 @@ inline void with_view(uint32_t V, const void* U, int is_f32, Fn&& fn) {
 
 template <typename R>
 __device__ __forceinline__ void store_link(R* __restrict__ p, uint32_t V, int mu, uint32_t s, const Mat<R>& m) {
+    using P = typename Pair2<R>::type;
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
-        p[(size_t)((mu * 9 + e) * 2) * V + s] = m.r[e];
-        p[(size_t)((mu * 9 + e) * 2 + 1) * V + s] = m.i[e];
+        P v;
+        v.x = m.r[e];
+        v.y = m.i[e];
+        reinterpret_cast<P*>(p + (size_t)(mu * 9 + e) * 2 * V)[s] = v;
     }
 }
 

@@ -30,8 +30,9 @@ __global__ void __launch_bounds__(256) import_kernel(Lat L, const Rin* __restric
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const int64_t off = 2 * (base + a * st_a + b * st_b);
-            dst[(size_t)((mu * 9 + a * 3 + b) * 2) * L.V + s] = (Rout)src[off];
-            dst[(size_t)((mu * 9 + a * 3 + b) * 2 + 1) * L.V + s] = (Rout)src[off + 1];
+            Rout* d = dst + ((size_t)(mu * 9 + a * 3 + b) * L.V + s) * 2;
+            d[0] = (Rout)src[off];
+            d[1] = (Rout)src[off + 1];
         }
 }
 
@@ -56,8 +57,9 @@ __global__ void __launch_bounds__(256) export_kernel(Lat L, const Rin* __restric
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const int64_t off = 2 * (base + a * st_a + b * st_b);
-            dst[off] = (Rout)src[(size_t)((mu * 9 + a * 3 + b) * 2) * L.V + s];
-            dst[off + 1] = (Rout)src[(size_t)((mu * 9 + a * 3 + b) * 2 + 1) * L.V + s];
+            const Rin* q = src + ((size_t)(mu * 9 + a * 3 + b) * L.V + s) * 2;
+            dst[off] = (Rout)q[0];
+            dst[off + 1] = (Rout)q[1];
         }
 }
 
@@ -73,8 +75,9 @@ __global__ void __launch_bounds__(256) random_kernel(Lat L, uint64_t seed, doubl
     const Mat<double> U = random_link(seed, sref + (uint64_t)L.V * mu, eps);
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
-        dst[(size_t)((mu * 9 + e) * 2) * L.V + s] = (R)U.r[e];
-        dst[(size_t)((mu * 9 + e) * 2 + 1) * L.V + s] = (R)U.i[e];
+        R* d = dst + ((size_t)(mu * 9 + e) * L.V + s) * 2;
+        d[0] = (R)U.r[e];
+        d[1] = (R)U.i[e];
     }
 }
 

@@ -1,8 +1,8 @@
 """Device-resident gauge fields and the device entry points of libflue_hip.so.
 
-A :class:`GaugeField` owns one torch tensor holding the library's SoA layout (72
-planes of V reals: direction mu, colour a, b, real/imaginary; site index
-x + NX*(y + NY*(z + NZ*t))) in FP64 or FP32.  Its methods call the C ABI on the
+A :class:`GaugeField` owns one torch tensor holding the library's SoA layout (36
+complex planes, direction mu and colour a, b, of V sites with adjacent real and
+imaginary parts; site index x + NX*(y + NY*(z + NZ*t))) in FP64 or FP32.  Its methods call the C ABI on the
 current torch stream and keep the results on the device until asked for.
 
 Directions and coordinates follow the reference (src/wloops.F90): mu = 1 is time,

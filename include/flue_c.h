@@ -16,9 +16,9 @@
  *    copied to the device, re-laid out and measured there.
  *
  * 2. Device entry points (flue_*) on device-resident fields in the library's
- *    SoA layout: 72 planes (direction mu, colour a, b, re/im) of V reals, site
- *    s = x + NX*(y + NY*(z + NZ*t)), plane ((mu*3 + a)*3 + b)*2 + ri, FP64 or
- *    FP32.  They are asynchronous on the given hipStream_t (NULL = default
+ *    SoA layout: 36 complex planes (direction mu, colour a, b) of V sites, each
+ *    site an adjacent (re, im) pair: real index (((mu*3 + a)*3 + b)*V + s)*2 + ri
+ *    with s = x + NX*(y + NY*(z + NZ*t)), FP64 or FP32 (72 V reals in all).  They are asynchronous on the given hipStream_t (NULL = default
  *    stream), write results to device memory and never synchronise.  Lattice
  *    sums are deterministic (fixed reduction order) and use compensated FP64
  *    partial sums; `ws` is a device workspace of at least

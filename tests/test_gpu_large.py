@@ -17,7 +17,8 @@ VS = 16 ** 3
 
 def tiled(FL, g, k):
     NT = g.dims[0]
-    data = g.data.view(72, NT, VS).repeat(1, k, 1).reshape(-1)
+    # 36 complex planes of V (re, im) pairs; a time slice is 2 VS contiguous reals of a plane
+    data = g.data.view(36, NT, 2 * VS).repeat(1, k, 1).reshape(-1)
     return FL.GaugeField((NT * k,) + g.dims[1:], g.precision, _data=data)
 
 
@@ -62,8 +63,8 @@ def test_unbuffered_flow_step_acts_slice_by_slice(FL):
     try:
         g.wilson_flow(0.02, 1, use_graph=False)
         big.wilson_flow(0.02, 1, use_graph=False)
-        a = g.data.view(72, SMALL[0], VS)
-        b = big.data.view(72, SMALL[0] * k, VS)
+        a = g.data.view(36, SMALL[0], 2 * VS)
+        b = big.data.view(36, SMALL[0] * k, 2 * VS)
         for c in (0, k // 2, k - 1):
             assert float((b[:, c * SMALL[0]:(c + 1) * SMALL[0]] - a).abs().max()) < 1e-13
     finally:
