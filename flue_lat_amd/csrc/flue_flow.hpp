@@ -8,7 +8,7 @@ namespace flue {
 namespace {
 
 // staple sum Sigma_mu(x): U_mu(x) Sigma is the sum of the six plaquettes through the link
-template <bool UNROLL = false, typename FV>
+template <typename FV>
 __device__ __forceinline__ Mat<typename FV::real> staple_sum(const Lat& L, const FV& f, uint32_t s, const int c[4],
                                                              int mu) {
     using R = typename FV::real;
@@ -17,8 +17,9 @@ __device__ __forceinline__ Mat<typename FV::real> staple_sum(const Lat& L, const
     for (int e = 0; e < 9; ++e) St.r[e] = St.i[e] = R(0);
     int cm[4] = {c[0], c[1], c[2], c[3]};
     const uint32_t sm = step_fwd(L, s, cm, mu);  // x + mu
-    auto staple = [&](int nu) {
-        if (nu == mu) return;
+#pragma unroll 1
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
         // up: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
         int cn[4] = {c[0], c[1], c[2], c[3]};
         const uint32_t sn = step_fwd(L, s, cn, nu);
@@ -35,13 +36,6 @@ __device__ __forceinline__ Mat<typename FV::real> staple_sum(const Lat& L, const
             St.r[e] += up.r[e] + dn.r[e];
             St.i[e] += up.i[e] + dn.i[e];
         }
-    };
-    if constexpr (UNROLL) {
-#pragma unroll
-        for (int nu = 0; nu < 4; ++nu) staple(nu);
-    } else {
-#pragma unroll 1
-        for (int nu = 0; nu < 4; ++nu) staple(nu);
     }
     return St;
 }
@@ -113,13 +107,10 @@ __device__ __forceinline__ void flow_site(const Lat& L, const FV& f, typename FV
     using R = typename FV::real;
     int c[4];
     site_coords(L, s, c);
-    // FP32 with buffer loads: the directions and staples are unrolled, so one direction's
-    // link loads overlap the previous direction's FP64 update (fits in 2 waves/SIMD; in
-    // FP64 the unrolled form would spill)
-    constexpr bool kUnroll = sizeof(R) == 4 && FV::buffered;
-    auto update = [&](int mu) {
+#pragma unroll 1
+    for (int mu = 0; mu < 4; ++mu) {
         const Mat<R> U = load_link(f, mu, s);
-        const Mat<double> Z = to_double(proj_ta(mul(U, staple_sum<kUnroll>(L, f, s, c, mu))));
+        const Mat<double> Z = to_double(proj_ta(mul(U, staple_sum(L, f, s, c, mu))));
         double z[8], a[8];
         su3_to8(Z, z);
 #pragma unroll
@@ -132,13 +123,6 @@ __device__ __forceinline__ void flow_site(const Lat& L, const FV& f, typename FV
         Mat<double> W = mul(su3_exp(su3_from8(a)), to_double(U));
         if (reunit) fix_su3(W);
         store_link(wout, L.V, mu, s, from_double<R>(W));
-    };
-    if constexpr (kUnroll) {
-#pragma unroll
-        for (int mu = 0; mu < 4; ++mu) update(mu);
-    } else {
-#pragma unroll 1
-        for (int mu = 0; mu < 4; ++mu) update(mu);
     }
 }
 
