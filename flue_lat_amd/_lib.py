@@ -71,6 +71,7 @@ SIGNATURES = {
     "flue_path_trace": [_P, _P, _I, _P, _I, _I, _P, _P, _SZ, _P],
     "flue_generic_path": [_P, _P, _I, _P, _P, _I, _P, _P],
     "flue_wilson_loops": [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _SZ, _P],
+    "flue_wilson_loops_slices": [_P, _P, _I, _I, _I, _I, _I, _P, _P, _SZ, _P],
     "flue_fmunu": [_P, _P, _I, _I, _I, _I, _P, _I, _P, _P],
     "flue_magnetic": [_P, _P, _I, _P, _P, _SZ, _P],
     "flue_magnetic_ex": [_P, _P, _I, _I, _P, _P, _SZ, _P],

@@ -294,7 +294,7 @@ size_t flue_workspace_bytes(const int32_t dims[4]) {
     nb = std::max(nb, path_blocks(L, -1));
     nb = std::max(nb, magnetic_blocks(L, 0));
     nb = std::max(nb, magnetic_blocks(L, 1));
-    return std::max(partial_bytes(nb), wloop_partial_bytes(L, -1) + 256);
+    return std::max({partial_bytes(nb), wloop_partial_bytes(L, -1) + 256, wloop_partial_bytes(L, kEverySlice) + 256});
 }
 
 int flue_import_field(const int32_t dims[4], const void* src, int32_t src_type, const int64_t strides[7], void* dst,
@@ -397,6 +397,21 @@ int flue_wilson_loops(const int32_t dims[4], const void* U, int32_t prec, int32_
     const double scale = 1.0 / (double)(ts >= 0 ? L.Vs : L.V);
     FLUE_TRY(launch_wloop_table(L, U, prec == FLUE_F32, mu - 1, nu - 1, Rmax, Tmax, ts, (double*)w, d_out, scale,
                                 S(stream)));
+    return FLUE_OK;
+}
+
+int flue_wilson_loops_slices(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, int32_t nu,
+                             int32_t Rmax, int32_t Tmax, double* d_out, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !d_out) return FLUE_ENULL;
+    if (check_prec(prec) || mu < 1 || mu > 4 || nu < 1 || nu > 4 || mu == nu) return FLUE_EINVAL;
+    if (Rmax < 1 || Rmax > kMaxLoopRT || Tmax < 1 || Tmax > kMaxLoopRT) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, wloop_partial_bytes(L, kEverySlice), &w))) return rc;
+    FLUE_TRY(launch_wloop_table(L, U, prec == FLUE_F32, mu - 1, nu - 1, Rmax, Tmax, kEverySlice, (double*)w, d_out,
+                                1.0 / (double)L.Vs, S(stream)));
     return FLUE_OK;
 }
 

@@ -31,12 +31,14 @@ __global__ void __launch_bounds__(kBS) wloop_table_kernel(Lat L, FV f, int mu, i
                                                           int tslice, uint32_t nb, double* __restrict__ partials) {
     using R_ = typename FV::real;
     __shared__ double sh[kMaxLoopRT * kMaxLoopRT][kWaves];
+    // tslice >= 0: that time slice; -1: every site; kEverySlice: one slice per blockIdx.y
+    const int ts = tslice == kEverySlice ? (int)blockIdx.y : tslice;
     const uint32_t blk = xcd_block(blockIdx.x, nb);
     const uint32_t idx = blk * kBS + threadIdx.x;
-    const uint32_t n = tslice >= 0 ? L.Vs : L.V;
+    const uint32_t n = ts >= 0 ? L.Vs : L.V;
     const bool live = idx < n;
     // lanes past the end work on site 0 (in bounds) and contribute nothing
-    const uint32_t s = live ? (tslice >= 0 ? idx + L.Vs * (uint32_t)tslice : idx) : 0u;
+    const uint32_t s = live ? (ts >= 0 ? idx + L.Vs * (uint32_t)ts : idx) : 0u;
     int c[4];
     site_coords(L, s, c);
     const int cmu = c[mu], cnu = c[nu];
@@ -91,42 +93,47 @@ __global__ void __launch_bounds__(kBS) wloop_table_kernel(Lat L, FV f, int mu, i
         DD r = {0.0, 0.0};
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) r = dd_add(r, DD{sh[k][w], 0.0});
-        double* p = partials + 2 * ((size_t)k * nb + blk);
+        double* p = partials + 2 * (((size_t)k * gridDim.y + blockIdx.y) * nb + blk);
         p[0] = r.hi;
         p[1] = r.lo;
     }
 }
 
-// block k of the grid sums table entry (R, T) = (k / Tmax + 1, k % Tmax + 1) over the nb
-// block partials in a fixed order and writes out[k] = scale * sum
+// block (k, j) of the grid sums table entry (R, T) = (k / Tmax + 1, k % Tmax + 1) of slice
+// group j (of gridDim.y) over its nb block partials in a fixed order and writes
+// out[k * gridDim.y + j] = scale * sum
 __global__ void __launch_bounds__(kBS) wloop_finalize_kernel(const double* __restrict__ partials, uint32_t nb,
                                                              int Tmax, double* __restrict__ out, double scale) {
     const int k = blockIdx.x;
     const int e = (k / Tmax) * kMaxLoopRT + (k % Tmax);
-    const double* p = partials + 2 * (size_t)e * nb;
+    const double* p = partials + 2 * (((size_t)e * gridDim.y + blockIdx.y) * nb);
     DD acc = {0.0, 0.0};
     for (uint32_t i = threadIdx.x; i < nb; i += kBS) acc = dd_add(acc, DD{p[2 * i], p[2 * i + 1]});
     const DD r = block_sum<kBS>(acc);
-    if (threadIdx.x == 0) out[k] = (r.hi + r.lo) * scale;
+    if (threadIdx.x == 0) out[(size_t)k * gridDim.y + blockIdx.y] = (r.hi + r.lo) * scale;
 }
 
 }  // namespace
 
-uint32_t wloop_blocks(const Lat& L, int tslice) { return ((tslice >= 0 ? L.Vs : L.V) + kBS - 1) / kBS; }
+// blocks per slice group: a single slice (tslice >= 0 or kEverySlice) or the whole lattice
+uint32_t wloop_blocks(const Lat& L, int tslice) { return ((tslice != -1 ? L.Vs : L.V) + kBS - 1) / kBS; }
 
 size_t wloop_partial_bytes(const Lat& L, int tslice) {
-    return (size_t)kMaxLoopRT * kMaxLoopRT * wloop_blocks(L, tslice) * 2 * sizeof(double);
+    const size_t groups = tslice == kEverySlice ? (size_t)L.n[0] : 1;
+    return (size_t)kMaxLoopRT * kMaxLoopRT * groups * wloop_blocks(L, tslice) * 2 * sizeof(double);
 }
 
 hipError_t launch_wloop_table(const Lat& L, const void* U, int is_f32, int mu, int nu, int Rmax, int Tmax,
                               int tslice, double* partials, double* out, double scale, hipStream_t st) {
     const uint32_t nb = wloop_blocks(L, tslice);
+    const uint32_t groups = tslice == kEverySlice ? (uint32_t)L.n[0] : 1u;
     with_view(L.V, U, is_f32, [&](auto f) {
-        wloop_table_kernel<decltype(f)><<<nb, kBS, 0, st>>>(L, f, mu, nu, Rmax, Tmax, tslice, nb, partials);
+        wloop_table_kernel<decltype(f)>
+            <<<dim3(nb, groups), kBS, 0, st>>>(L, f, mu, nu, Rmax, Tmax, tslice, nb, partials);
     });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    wloop_finalize_kernel<<<Rmax * Tmax, kBS, 0, st>>>(partials, nb, Tmax, out, scale);
+    wloop_finalize_kernel<<<dim3(Rmax * Tmax, groups), kBS, 0, st>>>(partials, nb, Tmax, out, scale);
     return hipGetLastError();
 }
 

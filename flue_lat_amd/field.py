@@ -188,6 +188,18 @@ class GaugeField:
                   "flue_wilson_loops")
         return out.cpu().numpy().reshape(int(Rmax), int(Tmax))
 
+    def wilson_loops_slices(self, mu, nu, Rmax=6, Tmax=6):
+        """W[R-1, T-1, t-1] = wmunutau(t, mu, nu, R, T) for every time slice t, in one pass
+        (the static-potential table W(R, T, t); src/wloops.F90:567-677)."""
+        NT = self.dims[0]
+        out = torch.empty(int(Rmax) * int(Tmax) * NT, dtype=torch.float64, device=self.device)
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_wilson_loops_slices(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu),
+                                                      int(Rmax), int(Tmax), _ptr(out), _ptr(ws), ws.numel(),
+                                                      _stream()), "flue_wilson_loops_slices")
+        return out.cpu().numpy().reshape(int(Rmax), int(Tmax), NT)
+
     def generic_path(self, coordBase, path):
         """genericPath at one 1-based base point: the transposed ordered product (src/wloops.F90:20-57)."""
         out = torch.empty(18, dtype=torch.float64, device=self.device)

@@ -182,6 +182,12 @@ int flue_path_trace(const int32_t dims[4], const void* U, int32_t prec, const in
 int flue_wilson_loops(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, int32_t nu, int32_t Rmax,
                       int32_t Tmax, int32_t tau, double* d_out, void* ws, size_t ws_bytes, void* stream);
 
+/* the same table for every time slice in one pass (static potential W(R, T, t)):
+ * d_out[((R-1)*Tmax + (T-1))*NT + (t-1)] = wmunutau(data, t, mu, nu, R, T) for t = 1..NT
+ * (src/wloops.F90:567-677) */
+int flue_wilson_loops_slices(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, int32_t nu,
+                             int32_t Rmax, int32_t Tmax, double* d_out, void* ws, size_t ws_bytes, void* stream);
+
 /* genericPath at one 1-based base point, transposed, as 9 complex128 in Fortran order
  * written to device memory d_out */
 int flue_generic_path(const int32_t dims[4], const void* U, int32_t prec, const int32_t coordBase[4],

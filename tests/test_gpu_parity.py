@@ -176,6 +176,19 @@ def test_wilson_loop_table_fp64(FL, small, mu, nu):
                 assert abs(Wt[R - 1, T - 1] - O.wmunutau(small, 2, mu, nu, R, T)) < 1e-13, (R, T)
 
 
+def test_wilson_loop_slices(FL, small):
+    g = gfield(FL, small)
+    W = g.wilson_loops_slices(2, 3, 4, 5)
+    assert W.shape == (4, 5, small.shape[0])
+    for (R, T) in ((1, 1), (4, 5), (2, 3)):
+        for t in (1, 3, 6):
+            assert abs(W[R - 1, T - 1, t - 1] - O.wmunutau(small, t, 2, 3, R, T)) < 1e-13, (R, T, t)
+    # the slice averages average to the whole-lattice table
+    assert np.abs(W.mean(axis=2) - g.wilson_loops(2, 3, 4, 5)).max() < 1e-13
+    W1 = g.wilson_loops_slices(1, 4, 6, 6)      # loops reaching along time too
+    assert abs(W1[5, 5, 4] - O.wmunutau(small, 5, 1, 4, 6, 6)) < 1e-13
+
+
 def test_wilson_loop_table_fp32_and_errors(FL, small):
     W = gfield(FL, small, "fp32").wilson_loops(3, 2, 3, 6)
     for R in range(1, 4):
