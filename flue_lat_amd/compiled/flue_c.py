@@ -3,9 +3,9 @@
 Mirrors the module numpy.f2py builds from c_wrapper/FLUE_c.f90 with --lower
 (python/meson.build:55-67): lattice dimensions are inferred from the array shapes,
 non-Fortran-ordered inputs are copied to Fortran order, and the ``intent(out)``
-arguments come back as a tuple in declaration order.  A nonzero status from the
+arguments come back as a tuple in argument-list order.  A nonzero status from the
 library raises FlueError (the Fortran routines would stop or run into undefined
-behaviour instead); the propagator routines raise NotImplementedError.
+behaviour instead).
 
 The field ``data`` is a complex array of shape (NT, NX, NY, NZ, 4, 3, 3) indexed
 [t, x, y, z, mu, a, b] = (U_mu(x))_ab, mu = 0 time (src/wloops.F90:21).
@@ -16,9 +16,6 @@ import numpy as np
 
 from .. import _lib
 from .._lib import check
-
-_FLUE_ENOTIMPL = 8
-
 
 def _c128(a, shape=None):
     a = np.asfortranarray(np.asarray(a), dtype=np.complex128)
@@ -64,24 +61,66 @@ def get_qhat_c(coord, shape, a=None):
     return q
 
 
-def _notimpl(name):
-    raise NotImplementedError(f"{name}: the gluon propagator is outside the scope of flue_lat_amd")
-
-
 def scalargluonprop_c(U, xi):
-    _notimpl("scalargluonprop_c")
+    """scalarGluonProp_c (c_wrapper/FLUE_c.f90:9-16): U of shape (NT,NS,NS,NS,4,3,3) with
+    NT <= NS -> D of shape (NS,NS,NS,NS), computed on the GPU."""
+    U = _field(U)
+    NT, NS = U.shape[0], U.shape[1]
+    if U.shape[2] != NS or U.shape[3] != NS:
+        raise ValueError("U must have shape (NT,NS,NS,NS,4,3,3)")
+    D = np.zeros((NS,) * 4, np.float64, order="F")
+    check(_lib.lib().scalargluonprop_c(_ri(NS), _ri(NT), _p(U), ctypes.byref(ctypes.c_double(xi)), _p(D)),
+          "scalargluonprop_c")
+    return D
 
 
 def calc_mom_space_scalard_c(U, mu_start, xi):
-    _notimpl("calc_mom_space_scalard_c")
+    """calc_mom_space_scalarD_c (c_wrapper/FLUE_c.f90:18-29): U is the Fourier-transformed
+    field declared U(0:NT, 0:NS, 0:NS, 0:NS, 0:3, 0:2, 0:2), so NT and NS are one less than
+    its extents -> D of shape (NT//2+1, NS//2+1, NS//2+1, NS//2+1), computed on the GPU."""
+    U = _field(U)
+    NT, NS = U.shape[0] - 1, U.shape[1] - 1
+    if U.shape[2] != NS + 1 or U.shape[3] != NS + 1:
+        raise ValueError("U must have shape (NT+1,NS+1,NS+1,NS+1,4,3,3)")
+    D = np.zeros((NT // 2 + 1,) + (NS // 2 + 1,) * 3, np.complex128, order="F")
+    check(_lib.lib().calc_mom_space_scalard_c(_ri(NS), _ri(NT), _p(U), _p(D), _ri(mu_start),
+                                              ctypes.byref(ctypes.c_double(xi))), "calc_mom_space_scalard_c")
+    return D
 
 
-def q_average_c(*args):
-    _notimpl("q_average_c")
+def q_average_c(NQ, q, D, q_slices):
+    """Q_Average_c (c_wrapper/FLUE_c.f90:38-47) -> (DOUT, QOUT, Qcount), DOUT and QOUT of
+    NQ + 1 values."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    n = int(NQ)
+    if q.size < n or D.size < n:
+        raise ValueError("q and D need NQ values")
+    dout = np.zeros(max(n, 0) + 1)
+    qout = np.zeros(max(n, 0) + 1)
+    cnt = ctypes.c_int32(0)
+    check(_lib.lib().q_average_c(_ri(n), _p(q), _p(D), _ri(q_slices), _p(dout), _p(qout), ctypes.byref(cnt)),
+          "q_average_c")
+    return dout, qout, cnt.value
 
 
-def cone_cut_c(*args):
-    _notimpl("cone_cut_c")
+def cone_cut_c(NQ, radius, Q, D, D4, NT, NS, angleIN, xiIN, IRCutIN, IRRadiusIN):
+    """cone_cut_c (c_wrapper/FLUE_c.f90:58-71) -> (QOUT, Dout, D4out, qcount)."""
+    n = int(NQ)
+    Q = np.asfortranarray(Q, dtype=np.float64)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    D4 = np.ascontiguousarray(D4, dtype=np.float64)
+    if Q.shape != (n, 4) or D.size < n or D4.size < n:
+        raise ValueError("Q must be (NQ, 4) and D, D4 hold NQ values")
+    qo = np.zeros((n, 4), np.float64, order="F")
+    do = np.zeros(n)
+    d4o = np.zeros(n)
+    cnt = ctypes.c_int32(0)
+    dbl = lambda v: ctypes.byref(ctypes.c_double(float(v)))  # noqa: E731
+    check(_lib.lib().cone_cut_c(_ri(n), _ri(radius), _p(Q), _p(D), _p(D4), _ri(NT), _ri(NS), dbl(angleIN),
+                                dbl(xiIN), dbl(IRCutIN), dbl(IRRadiusIN), _p(qo), _p(do), _p(d4o),
+                                ctypes.byref(cnt)), "cone_cut_c")
+    return qo, do, d4o, cnt.value
 
 
 def periodcoord_c(coord, datashape):

@@ -250,20 +250,81 @@ int get_qhat_c(const double coord[4], const int32_t shape[4], double qhat[4], co
     return FLUE_OK;
 }
 
-int scalargluonprop_c(const int32_t*, const int32_t*, const flue_c128*, const double*, double*) {
-    return FLUE_ENOTIMPL;
+int q_average_c(const int32_t* NQ, const double* q, const double* D, const int32_t* q_slices, double* DOUT,
+                double* QOUT, int32_t* Qcount) {
+    if (!NQ || !q || !D || !q_slices || !DOUT || !QOUT || !Qcount) return FLUE_ENULL;
+    const int32_t n = *NQ;
+    if (n < 1 || *q_slices < 1) return FLUE_EINVAL;
+    for (int32_t i = 0; i < n; ++i)
+        if (std::isnan(q[i])) return FLUE_EINVAL;
+    std::fill(DOUT, DOUT + n + 1, 0.0);
+    std::fill(QOUT, QOUT + n + 1, 0.0);
+    const double maxq = *std::max_element(q, q + n);
+    const double dq = maxq / (double)*q_slices;
+    // a negative step never reaches the exit test of the reference's loop
+    if (!(dq >= 0.0)) return FLUE_EINVAL;
+    double lo = 0.0;
+    int32_t count = 1;
+    for (;;) {
+        const double hi = lo + dq;
+        double sum = 0.0;
+        int32_t k = 0;
+        for (int32_t i = 0; i < n; ++i)
+            if (q[i] > lo && q[i] < hi) {
+                sum += q[i];
+                ++k;
+            }
+        // open bins (lo, lo + dq) are disjoint and each kept one holds a value: count <= n + 1
+        if (k > 0) {
+            QOUT[count - 1] = sum / (double)k;
+            DOUT[count - 1] = D[0];
+            ++count;
+        }
+        lo = hi;
+        if (lo + dq >= maxq) break;
+    }
+    *Qcount = count;
+    return FLUE_OK;
 }
-int calc_mom_space_scalard_c(const int32_t*, const int32_t*, const flue_c128*, flue_c128*, const int32_t*,
-                             const double*) {
-    return FLUE_ENOTIMPL;
-}
-int q_average_c(const int32_t*, const double*, const double*, const int32_t*, double*, double*, int32_t*) {
-    return FLUE_ENOTIMPL;
-}
-int cone_cut_c(const int32_t*, const int32_t*, const double*, const double*, const double*, const int32_t*,
-               const int32_t*, const double*, const double*, const double*, const double*, double*, double*,
-               double*, int32_t*) {
-    return FLUE_ENOTIMPL;
+
+int cone_cut_c(const int32_t* NQ, const int32_t* radius, const double* Q, const double* D, const double* D4,
+               const int32_t* NT, const int32_t* NS, const double* angleIN, const double* xiIN,
+               const double* IRCutIN, const double* IRRadiusIN, double* QOUT, double* Dout, double* D4out,
+               int32_t* qcount) {
+    (void)radius;  // IRRadiusIN is always given here, and radius only stands in for it when absent
+    (void)NT;
+    (void)NS;
+    if (!NQ || !Q || !D || !D4 || !angleIN || !xiIN || !IRCutIN || !IRRadiusIN || !QOUT || !Dout || !D4out || !qcount)
+        return FLUE_ENULL;
+    const int64_t n = *NQ;
+    if (n < 0) return FLUE_EINVAL;
+    const double angle = *angleIN, xi = *xiIN, ircut = *IRCutIN, irradius = *IRRadiusIN;
+    std::fill(QOUT, QOUT + 4 * n, 0.0);
+    std::fill(Dout, Dout + n, 0.0);
+    std::fill(D4out, D4out + n, 0.0);
+    int32_t kept = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        double v[4];
+        for (int i = 0; i < 4; ++i) v[i] = Q[k + n * i];
+        // the norm is taken before the anisotropy rescales the first component
+        const double norm = std::sqrt(((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + v[3] * v[3]);
+        v[0] *= xi;
+        double r = 0.0, theta = 0.0;
+        if (!(v[1] == v[0] && v[2] == v[0] && v[3] == v[0])) {
+            // angle to the (1,1,1,1)/2 diagonal
+            theta = std::acos((((0.5 * v[0] + 0.5 * v[1]) + 0.5 * v[2]) + 0.5 * v[3]) / norm);
+            r = norm * std::sin(theta);
+        }
+        const bool keep = norm <= ircut ? (r <= irradius) : (r <= irradius && theta < angle);
+        if (keep) {
+            for (int i = 0; i < 4; ++i) QOUT[kept + n * i] = Q[k + n * i];
+            Dout[kept] = D[k];
+            D4out[kept] = D4[k];
+            ++kept;
+        }
+    }
+    *qcount = kept;
+    return FLUE_OK;
 }
 
 int periodcoord_c(const int32_t coord[4], const int32_t dataShape[7], int32_t out[4]) {
@@ -441,6 +502,28 @@ int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, i
     if (layout == FLUE_OUT_STRIDED && !strides) return FLUE_ENULL;
     return hip_status(launch_fmunu(make_lat(dims), U, prec == FLUE_F32, mu - 1, nu - 1, variant, out,
                                    layout == FLUE_OUT_SOA, strides, S(stream)));
+}
+
+int flue_gluon_prop_mom(const int32_t dims[4], const void* Uq, int32_t prec, int32_t mu_start, double xi,
+                        const int32_t dext[4], void* d_D, const int64_t dstrides[4], void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!Uq || !d_D || !dext || !dstrides) return FLUE_ENULL;
+    if (check_prec(prec) || mu_start < 0) return FLUE_EINVAL;
+    for (int i = 0; i < 4; ++i)
+        if (dext[i] < 1 || dext[i] > dims[i] / 2 + 1) return FLUE_EINVAL;
+    return hip_status(launch_mom_scalar(make_lat(dims), Uq, prec == FLUE_F32, mu_start, xi, dext, dstrides,
+                                        (double*)d_D, S(stream)));
+}
+
+int flue_scalar_gluon_prop(const int32_t dims[4], const void* U, int32_t prec, double* d_D, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !d_D) return FLUE_ENULL;
+    if (check_prec(prec)) return FLUE_EINVAL;
+    // D(NS,NS,NS,NS) is written at every (t,x,y,z) of the field
+    if (dims[2] != dims[1] || dims[3] != dims[1] || dims[0] > dims[1]) return FLUE_EDIM;
+    return hip_status(launch_scalar_gluon(make_lat(dims), U, prec == FLUE_F32, dims[1], d_D, S(stream)));
 }
 
 int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_B, void* ws, size_t ws_bytes,
@@ -983,6 +1066,40 @@ int wmunutau_c(const int32_t* NT, const int32_t* NX, const int32_t* NY, const in
     if (!tau) return FLUE_ENULL;
     if (*tau < 1) return FLUE_EINVAL;
     return rect_host(NT, NX, NY, NZ, data, *tau, mu, nu, nMu, nNu, loopVal);
+}
+
+int calc_mom_space_scalard_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, flue_c128* D,
+                             const int32_t* mu_start, const double* xi) {
+    if (!NS || !NT || !U || !D || !mu_start || !xi) return FLUE_ENULL;
+    if (*NT < 0 || *NS < 0 || *NT == INT32_MAX || *NS == INT32_MAX) return FLUE_EDIM;
+    if (*mu_start < 0) return FLUE_EINVAL;
+    // U(0:NT, 0:NS, 0:NS, 0:NS, 0:3, 0:2, 0:2) and D(0:NT/2, 0:NS/2, 0:NS/2, 0:NS/2)
+    const int32_t Et = *NT + 1, Es = *NS + 1;
+    HostField h;
+    int rc;
+    if ((rc = load_host_field(&Et, &Es, &Es, &Es, U, h))) return rc;
+    const int32_t dext[4] = {*NT / 2 + 1, *NS / 2 + 1, *NS / 2 + 1, *NS / 2 + 1};
+    const int64_t dst[4] = {1, dext[0], (int64_t)dext[0] * dext[1], (int64_t)dext[0] * dext[1] * dext[2]};
+    const size_t n = (size_t)dst[3] * dext[3];
+    DevBuf d;
+    FLUE_TRY(hipMalloc(&d.p, n * sizeof(flue_c128)));
+    if ((rc = flue_gluon_prop_mom(h.dims, h.soa.p, FLUE_F64, *mu_start, *xi, dext, d.p, dst, nullptr))) return rc;
+    FLUE_TRY(hipMemcpy(D, d.p, n * sizeof(flue_c128), hipMemcpyDeviceToHost));
+    return FLUE_OK;
+}
+
+int scalargluonprop_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, const double* xi, double* D) {
+    if (!NS || !NT || !U || !xi || !D) return FLUE_ENULL;
+    if (*NT > *NS) return FLUE_EDIM;
+    HostField h;
+    int rc;
+    if ((rc = load_host_field(NT, NS, NS, NS, U, h))) return rc;
+    const size_t n = (size_t)h.dims[1] * h.dims[1] * h.dims[1] * h.dims[1];
+    DevBuf d;
+    FLUE_TRY(hipMalloc(&d.p, n * sizeof(double)));
+    if ((rc = flue_scalar_gluon_prop(h.dims, h.soa.p, FLUE_F64, (double*)d.p, nullptr))) return rc;
+    FLUE_TRY(hipMemcpy(D, d.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    return FLUE_OK;
 }
 
 }  // extern "C"

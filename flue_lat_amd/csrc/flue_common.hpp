@@ -1,11 +1,11 @@
 // flue_common.hpp -- device-side building blocks shared by the FLUE HIP kernels.
 //
-// Link storage ("SoA planes"): one real and one imaginary plane per direction and
-// colour element, each a contiguous array over the sites of the lattice:
-//     plane(mu, a, b, ri) = ((mu * 3 + a) * 3 + b) * 2 + ri,   element = plane * V + s
+// Link storage: 36 complex planes, one per direction and colour element, each a
+// contiguous array over the sites of (re, im) pairs:
+//     real index = (((mu * 3 + a) * 3 + b) * V + s) * 2 + ri
 // with the site index s = x + NX * (y + NY * (z + NZ * t)), so x is fastest and every
-// time slice is one contiguous block of Vs = NX*NY*NZ sites.  Directions use the
-// reference's numbering shifted to 0: mu = 0 is time, 1..3 are x, y, z
+// time slice is one contiguous block of Vs = NX*NY*NZ sites of each plane.  Directions
+// use the reference's numbering shifted to 0: mu = 0 is time, 1..3 are x, y, z
 // (src/wloops.F90:21 with mu=1 time).
 //
 // All matrices live in registers as 9 real + 9 imaginary parts, row-major (a*3+b).

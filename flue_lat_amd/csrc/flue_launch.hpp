@@ -48,6 +48,13 @@ size_t wloop_partial_bytes(const Lat& L, int tslice);
 hipError_t launch_wloop_table(const Lat& L, const void* U, int is_f32, int mu, int nu, int Rmax, int Tmax,
                               int tslice, double* partials, double* out, double scale, hipStream_t st);
 
+// scalar gluon propagator (k_gluon.hip): momentum-space D over 0 <= q < dext of the
+// transformed field Uq (lattice Lq of its extents), complex128 at strides dstr (t,x,y,z);
+// and the position-space form into D(NS,NS,NS,NS) (Fortran order, FP64)
+hipError_t launch_mom_scalar(const Lat& Lq, const void* Uq, int is_f32, int mu_start, double xi,
+                             const int32_t dext[4], const int64_t dstr[4], double* D, hipStream_t st);
+hipError_t launch_scalar_gluon(const Lat& L, const void* U, int is_f32, int NS, double* D, hipStream_t st);
+
 // partial sums per block of the first stage of a long fixed-order reduction
 constexpr uint32_t kFinalChunk = 2048;
 

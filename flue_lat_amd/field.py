@@ -221,6 +221,29 @@ class GaugeField:
                                         _ptr(out), FLUE_OUT_STRIDED, _i64(out.stride()), _stream()), "flue_fmunu")
         return out
 
+    def momentum_field(self):
+        """U_mu(q) = sum_x exp(-i q.x) U_mu(x) for every link element (numpy.fft.fftn's
+        convention), as a complex tensor (36, NT, NZ, NY, NX) laid out like the field with the
+        momenta in place of the sites."""
+        NT, NX, NY, NZ = self.dims
+        c = torch.view_as_complex(self.data.view(36, NT, NZ, NY, NX, 2))
+        # a multi-axis transform may hand back permuted strides: the kernel reads it dense
+        return torch.fft.fftn(c, dim=(1, 2, 3, 4)).contiguous()
+
+    def gluon_propagator(self, mu_start=1, xi=1.0):
+        """calc_mom_space_scalarD (src/gluonCalc/gluonProp.f90:112-250) of this field: the
+        4-D transform on the device, then the per-momentum colour algebra in one kernel.
+        Returns D as a complex128 tensor of shape (NT//2+1, NX//2+1, NY//2+1, NZ//2+1), the
+        reference's loop range over q."""
+        Uq = self.momentum_field()
+        dext = tuple(d // 2 + 1 for d in self.dims)
+        D = torch.empty(dext, dtype=torch.complex128, device=self.device)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_gluon_prop_mom(self._cdims, _ptr(Uq), self.prec, int(mu_start), float(xi),
+                                                 _i32(*dext), _ptr(D), _i64(D.stride()), _stream()),
+                  "flue_gluon_prop_mom")
+        return D
+
     def magnetic(self, sync=True, variant=0):
         """B = -(1/V) sum_x sum_i ReTr F_i^2 over the spatial planes (src/wloops.F90:479-544).
         variant selects the kernel form and occupancy target: 1-3 site-major, 4-6

@@ -81,13 +81,42 @@ int multiplymatmat_c(flue_c128 MM[9], const flue_c128 left[9], const flue_c128 r
  * the absent argument, momentum.f90:23) */
 int get_qhat_c(const double coord[4], const int32_t shape[4], double qhat[4], const double* a);
 
-/* propagator entries of c_wrapper/FLUE_c.f90:9-29,38-47,58-71 are outside this
- * library's scope and return FLUE_ENOTIMPL */
-int scalargluonprop_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, const double* xi, double* D);
+/* calc_mom_space_scalarD_c, c_wrapper/FLUE_c.f90:18-29 (src/gluonCalc/gluonProp.f90:112-250):
+ * scalar gluon propagator from the Fourier-transformed field
+ * U(0:NT, 0:NS, 0:NS, 0:NS, 0:3, 0:2, 0:2) (complex128, Fortran order) into
+ * D(0:NT/2, 0:NS/2, 0:NS/2, 0:NS/2).  Computed on the GPU (flue_gluon_prop_mom).  Where the
+ * reference's result is undefined: U(-q) is read at (E - q) mod E for the array extent E
+ * (the reference reads index E - q, one past the end of the dimension when q = 0,
+ * gluonProp.f90:177); the mu = 0 phase uses the t component of qhat (the reference reads
+ * qhat(0), outside its 1-based array, :185); momenta past D's extent, which the reference
+ * writes out of bounds when an extent E is even, are not computed.  The phase of
+ * directions 1..3 and the prefactor conditions (:216, :225) are the reference's. */
 int calc_mom_space_scalard_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, flue_c128* D,
                              const int32_t* mu_start, const double* xi);
+
+/* scalarGluonProp_c, c_wrapper/FLUE_c.f90:9-16 (src/gluonCalc/gluonProp.f90:17-110):
+ * position-space field U(NT,NS,NS,NS,4,3,3) into D(NS,NS,NS,NS), on the GPU
+ * (flue_scalar_gluon_prop).  Needs NT <= NS (the reference writes D at every (t,x,y,z)
+ * of the field: FLUE_EDIM otherwise).  D is overwritten (the reference adds into its
+ * intent(out) D without clearing it), entries with t > NT are 0, xi has no effect (it
+ * enters only for mu == 0, which the reference's 1-based direction loop never reaches),
+ * and the volume sum of identical terms (:69-73) is taken as V times the term. */
+int scalargluonprop_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, const double* xi, double* D);
+
+/* Q_Average_c, c_wrapper/FLUE_c.f90:38-47 (src/gluonAna/gpManip.f90:10-57), host code:
+ * averages q over the open bins (k dq, (k+1) dq), dq = max(q)/q_slices, until the next
+ * bin's upper edge reaches max(q); DOUT takes D(1) for every kept bin and Qcount is one
+ * more than the number of kept bins, as in the reference.  DOUT and QOUT hold NQ + 1
+ * values.  FLUE_EINVAL for NQ < 1, q_slices < 1, a NaN in q, or max(q) < 0 (on which the
+ * reference's loop does not end). */
 int q_average_c(const int32_t* NQ, const double* q, const double* D, const int32_t* q_slices, double* DOUT,
                 double* QOUT, int32_t* Qcount);
+
+/* cone_cut_c, c_wrapper/FLUE_c.f90:58-71 (src/gluonAna/gpManip.f90:59-163), host code:
+ * keeps the momenta Q(NQ,4) (Fortran order) within IRRadiusIN of the (1,1,1,1) diagonal
+ * and, above |q| = IRCutIN, at an angle below angleIN to it (the first component scaled
+ * by xiIN after |q| is taken); packs them with D and D4 into QOUT, Dout, D4out.  radius,
+ * NT and NS are accepted and unused, as in the reference when every optional is given. */
 int cone_cut_c(const int32_t* NQ, const int32_t* radius, const double* Q, const double* D, const double* D4,
                const int32_t* NT, const int32_t* NS, const double* angleIN, const double* xiIN,
                const double* IRCutIN, const double* IRRadiusIN, double* QOUT, double* Dout, double* D4out,
@@ -199,6 +228,21 @@ int flue_generic_path(const int32_t dims[4], const void* U, int32_t prec, const 
  * field precision (element a*3+b, re/im) */
 int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, int32_t nu, int32_t variant,
                void* out, int32_t layout, const int64_t strides[6], void* stream);
+
+/* calc_mom_space_scalarD (src/gluonCalc/gluonProp.f90:112-250) on a Fourier-transformed
+ * field Uq held in the SoA layout over its extents dims (the momenta q take the place of
+ * the sites): d_D (complex128) at element strides dstrides for (t, x, y, z) over
+ * 0 <= q < dext, dext[i] <= dims[i]/2 + 1 (the reference's loop range).  U(-q) is read at
+ * (dims - q) mod dims; the phase of direction mu >= 1 uses the momentum component of
+ * direction mu - 1 as the reference's qhat(mu) does, and that of mu = 0 the t component.
+ * mu_start >= 0; directions mu_start..3 are summed. */
+int flue_gluon_prop_mom(const int32_t dims[4], const void* Uq, int32_t prec, int32_t mu_start, double xi,
+                        const int32_t dext[4], void* d_D, const int64_t dstrides[4], void* stream);
+
+/* scalarGluonProp (src/gluonCalc/gluonProp.f90:17-110) on a position-space field with
+ * NT <= NX = NY = NZ = NS: d_D holds NS^4 doubles, D(t,x,y,z) in Fortran order, 0 for
+ * t >= NT */
+int flue_scalar_gluon_prop(const int32_t dims[4], const void* U, int32_t prec, double* d_D, void* stream);
 
 /* magnetic (src/wloops.F90:479-544): *d_B = -(1/V) sum_x sum_i ReTr F_i(x)^2 */
 int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_B, void* ws, size_t ws_bytes,

@@ -72,6 +72,10 @@ def _declare(L):
         "or_b2_grad": [_P, _P, _P],
         "or_fd_link": [_P, _P, _I, _D, _P, _I, _I, _D, _P],
         "or_wilson_flow": [_P, _P, _D, _I, _I],
+        "or_mom_scalar_d": [_P, _P, _P, _I, _D, _P],
+        "or_scalar_gluon_prop": [_P, _P, _D, _P],
+        "or_q_average": [_I, _P, _P, _I, _P, _P, _P],
+        "or_cone_cut": [_I, _P, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -298,3 +302,50 @@ def generators():
     if T_GEN is None:
         T_GEN = np.stack([-0.5j * np.asarray(gellmann(a)) for a in range(1, 9)])
     return T_GEN
+
+
+def mom_scalar_d(Uq, mu_start, xi, dext=None):
+    """calc_mom_space_scalarD on a Fourier-transformed field Uq of shape (Et,Ex,Ey,Ez,4,3,3);
+    D over 0 <= q < dext (default E // 2 + 1 per direction)."""
+    Uq = _field(Uq)
+    E = _dims(Uq)
+    dext = np.asarray(E // 2 + 1 if dext is None else dext, np.int32)
+    D = np.zeros(tuple(int(v) for v in dext), np.complex128, order="F")
+    _check(lib().or_mom_scalar_d(_ptr(Uq), _ptr(E), _ptr(dext), mu_start, xi, _ptr(D)), "mom_scalar_d")
+    return D
+
+
+def scalar_gluon_prop(U, xi):
+    """scalarGluonProp on a position-space field with NT <= NS; D of shape (NS,NS,NS,NS)."""
+    U = _field(U)
+    NS = U.shape[1]
+    D = np.zeros((NS,) * 4, np.float64, order="F")
+    _check(lib().or_scalar_gluon_prop(_ptr(U), _ptr(_dims(U)), xi, _ptr(D)), "scalar_gluon_prop")
+    return D
+
+
+def q_average(q, D, q_slices):
+    """Q_Average -> (DOUT, QOUT, Qcount) with NQ + 1 entries in DOUT, QOUT."""
+    q = np.ascontiguousarray(q, np.float64)
+    D = np.ascontiguousarray(D, np.float64)
+    n = q.shape[0]
+    dout = np.zeros(n + 1)
+    qout = np.zeros(n + 1)
+    cnt = np.zeros(1, np.int32)
+    _check(lib().or_q_average(n, _ptr(q), _ptr(D), q_slices, _ptr(dout), _ptr(qout), _ptr(cnt)), "q_average")
+    return dout, qout, int(cnt[0])
+
+
+def cone_cut(Q, D, D4, angle, xi, IRCut, IRRadius):
+    """cone_cut with every optional argument given -> (QOUT, DOUT, D4OUT, qcount)."""
+    Q = np.asfortranarray(Q, np.float64)
+    D = np.ascontiguousarray(D, np.float64)
+    D4 = np.ascontiguousarray(D4, np.float64)
+    n = Q.shape[0]
+    qo = np.zeros((n, 4), np.float64, order="F")
+    do = np.zeros(n)
+    d4o = np.zeros(n)
+    cnt = np.zeros(1, np.int32)
+    _check(lib().or_cone_cut(n, _ptr(Q), _ptr(D), _ptr(D4), angle, xi, IRCut, IRRadius, _ptr(qo), _ptr(do),
+                             _ptr(d4o), _ptr(cnt)), "cone_cut")
+    return qo, do, d4o, int(cnt[0])

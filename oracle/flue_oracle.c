@@ -1002,3 +1002,209 @@ int or_wilson_flow(cplx* data, const int32_t dims[4], double eps, int32_t nsteps
     free(acc);
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* Gluon propagator and its momentum-space analysis                    */
+/* (src/gluonCalc/gluonProp.f90, src/gluonAna/gpManip.f90)             */
+/* ------------------------------------------------------------------ */
+
+/* TrSub = left - right^+ - Tr(left - right^+)/3, the trace taken as Tr(TrSub * Ident) and
+ * the third subtracted from all nine entries, as the array expression of
+ * src/SU3MatrixOps.f90:94-105 does */
+static void traceless_conjg_subtract(cplx o[9], const cplx l[9], const cplx r[9]) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M3(o, i, j) = M3(l, i, j) - conj(M3(r, j, i));
+    const cplx tr = (M3(o, 0, 0) + M3(o, 1, 1)) + M3(o, 2, 2);
+    for (int i = 0; i < 9; ++i) o[i] = o[i] - tr / 3.0;
+}
+
+/* Gell-Mann components, src/SU3MatrixOps.f90:107-122 (com(3) keeps the imaginary part,
+ * com(8) has no 1/sqrt(3) on its first two terms) */
+static void colour_decomp(cplx com[8], const cplx A[9]) {
+    com[0] = creal(M3(A, 0, 1)) + creal(M3(A, 1, 0));
+    com[1] = -cimag(M3(A, 0, 1)) + cimag(M3(A, 1, 0));
+    com[2] = M3(A, 0, 0) - M3(A, 1, 1);
+    com[3] = creal(M3(A, 0, 2)) + creal(M3(A, 2, 0));
+    com[4] = -cimag(M3(A, 0, 2)) + cimag(M3(A, 2, 0));
+    com[5] = creal(M3(A, 1, 2)) + creal(M3(A, 2, 1));
+    com[6] = -cimag(M3(A, 1, 2)) + cimag(M3(A, 2, 1));
+    com[7] = creal(M3(A, 0, 0)) + creal(M3(A, 1, 1)) - (2.0 / sqrt(3.0)) * creal(M3(A, 2, 2));
+}
+
+/* calc_mom_space_scalarD, src/gluonCalc/gluonProp.f90:112-250, over the momenta
+ * 0 <= q < dext of the Fourier-transformed field Uq (extents E, Fortran order like a gauge
+ * field).  Readings where the reference's result is undefined:
+ *   - U(-q) is taken at (E - q) mod E; the reference reads index E - q (:177), one past
+ *     the dimension's end when a component of q is 0;
+ *   - the phase of direction mu (0-based, :185-186) uses qhat(mu) with qhat 1-based, as
+ *     the reference does, i.e. the momentum component of direction mu - 1; for mu = 0 the
+ *     reference reads qhat(0), outside the array, and this uses the t component.
+ * The prefactor keeps the reference's conditions (:216, :225), including qx counted twice. */
+int or_mom_scalar_d(const cplx* Uq, const int32_t E[4], const int32_t dext[4], int32_t mu_start, double xi,
+                    cplx* D) {
+    if (check_dims(E) || check_dims(dext) || mu_start < 0) return ORC_EINVAL;
+    for (int i = 0; i < 4; ++i)
+        if (dext[i] > E[i] / 2 + 1) return ORC_EINVAL;
+    const int64_t V = vol4(E);
+    const int64_t nD = vol4(dext);
+#pragma omp parallel for schedule(static)
+    for (int64_t o = 0; o < nD; ++o) {
+        int32_t q[4];
+        int64_t r = o;
+        for (int i = 0; i < 4; ++i) {
+            q[i] = (int32_t)(r % dext[i]);
+            r /= dext[i];
+        }
+        double qhat[4];
+        for (int i = 0; i < 4; ++i) qhat[i] = (2.0 * M_PI / 1.0) * (double)q[i] / (double)E[i];
+        int32_t qn[4];
+        for (int i = 0; i < 4; ++i) qn[i] = (E[i] - q[i]) % E[i];
+        const int64_t sp = site0(E, q[0], q[1], q[2], q[3]);
+        const int64_t sn = site0(E, qn[0], qn[1], qn[2], qn[3]);
+        cplx acc = 0.0;
+        for (int mu = mu_start; mu <= 3; ++mu) {
+            cplx Up[9], Un[9], Ap[9], An[9];
+            for (int b = 0; b < 3; ++b)
+                for (int a = 0; a < 3; ++a) {
+                    M3(Up, a, b) = Uq[sp + V * (mu + 4 * (a + 3 * b))];
+                    M3(Un, a, b) = Uq[sn + V * (mu + 4 * (a + 3 * b))];
+                }
+            traceless_conjg_subtract(Ap, Up, Un);
+            traceless_conjg_subtract(An, Un, Up);
+            const double th = qhat[mu >= 1 ? mu - 1 : 0];
+            const cplx pp = (0.5 * I) * cexp(-I * 0.5 * th);
+            const cplx pn = (0.5 * I) * cexp(I * 0.5 * th);
+            for (int i = 0; i < 9; ++i) {
+                Ap[i] = pp * Ap[i];
+                An[i] = pn * An[i];
+            }
+            cplx cp[8], cn[8];
+            colour_decomp(cp, Ap);
+            colour_decomp(cn, An);
+            double s = 0.0;
+            for (int a = 0; a < 8; ++a) s += creal(mu == 0 ? xi * xi * cp[a] * cn[a] : cp[a] * cn[a]);
+            acc += s;
+        }
+        const double Vd = (double)E[0] * (double)E[1] * (double)E[2] * (double)E[3];
+        double prefac;
+        if ((mu_start == 1 && q[1] + q[2] + q[1] < 1) || (mu_start == 0 && q[0] + q[1] + q[2] + q[3] + q[1] < 1))
+            prefac = 2.0 / (8.0 * Vd * 4.0);
+        else
+            prefac = 2.0 / (8.0 * Vd * 3.0);
+        D[o] = acc * prefac;
+    }
+    return ORC_OK;
+}
+
+/* scalarGluonProp, src/gluonCalc/gluonProp.f90:17-110, on a position-space field with
+ * NT <= NS (the reference writes D(NS,NS,NS,NS) at every (t,x,y,z) of the field).  D starts
+ * at zero (the reference accumulates into its intent(out) D without clearing it, :103-124),
+ * and the sum over the volume of identical terms (:69-73) is carried out term by term as
+ * the reference writes it. */
+int or_scalar_gluon_prop(const cplx* data, const int32_t dims[4], double xi, double* D) {
+    (void)xi; /* enters only for mu == 0, which the 1-based direction loop never reaches */
+    if (check_dims(dims)) return ORC_EINVAL;
+    const int32_t NT = dims[0], NS = dims[1];
+    if (dims[2] != NS || dims[3] != NS || NT > NS) return ORC_EINVAL;
+    const int64_t V = vol4(dims);
+    const int32_t dD[4] = {NS, NS, NS, NS};
+    memset(D, 0, sizeof(double) * (size_t)vol4(dD));
+    for (int mu = 1; mu <= 4; ++mu) {
+#pragma omp parallel for schedule(static)
+        for (int64_t s = 0; s < V; ++s) {
+            int32_t c[4];
+            site_coord(dims, s, c);
+            double qhat[4];
+            for (int i = 0; i < 4; ++i) qhat[i] = (2.0 * M_PI / 1.0) * (double)c[i] / (double)dims[i];
+            cplx Umu[9], T1[9], T2[9], A[9];
+            load_link(data, dims, c, mu, Umu);
+            for (int i = 0; i < 9; ++i) T1[i] = T2[i] = 0.0;
+            const double ph = (double)c[mu - 1] * qhat[mu - 1];
+            const cplx e1 = cexp(-I * ph), e2 = cexp(I * ph);
+            for (int64_t k = 0; k < V; ++k)
+                for (int i = 0; i < 9; ++i) {
+                    T1[i] += e1 * Umu[i];
+                    T2[i] += e2 * Umu[i];
+                }
+            traceless_conjg_subtract(A, T2, T1);
+            const cplx p1 = (0.5 * I) * cexp(-I * qhat[mu - 1] * 0.5);
+            const cplx p2 = (0.5 * I) * cexp(I * qhat[mu - 1] * 0.5);
+            for (int i = 0; i < 9; ++i) A[i] = p2 * (p1 * A[i]);
+            cplx com[8];
+            colour_decomp(com, A);
+            cplx cs = 0.0;
+            for (int a = 0; a < 8; ++a) cs += com[a] * com[a];
+            D[site0(dD, c[0] - 1, c[1] - 1, c[2] - 1, c[3] - 1)] += creal(cs);
+        }
+    }
+    const double f = (1.0 / 3.0) * (1.0 / 8.0) * (double)V;
+    for (int64_t i = 0; i < vol4(dD); ++i) D[i] = f * D[i];
+    return ORC_OK;
+}
+
+/* Q_Average, src/gluonAna/gpManip.f90:10-57 (DOUT takes D(1) for every slice, as there) */
+int or_q_average(int32_t NQ, const double* q, const double* D, int32_t q_slices, double* DOUT, double* QOUT,
+                 int32_t* Qcount) {
+    if (NQ < 1 || q_slices < 1) return ORC_EINVAL;
+    for (int32_t i = 0; i < NQ; ++i)
+        if (isnan(q[i])) return ORC_EINVAL;
+    for (int32_t i = 0; i <= NQ; ++i) DOUT[i] = QOUT[i] = 0.0;
+    double maxq = q[0];
+    for (int32_t i = 1; i < NQ; ++i)
+        if (q[i] > maxq) maxq = q[i];
+    const double dq = maxq / (double)q_slices;
+    if (!(dq >= 0.0)) return ORC_EINVAL; /* the reference loops without end */
+    double q_pos = 0.0;
+    int32_t qcount = 1;
+    for (;;) {
+        double sum = 0.0;
+        int32_t n = 0;
+        for (int32_t i = 0; i < NQ; ++i)
+            if (q[i] > q_pos && q[i] < q_pos + dq) {
+                sum += q[i];
+                ++n;
+            }
+        if (n > 0) {
+            QOUT[qcount - 1] = sum / (double)n;
+            DOUT[qcount - 1] = D[0];
+            ++qcount;
+        }
+        q_pos = q_pos + dq;
+        if (q_pos + dq >= maxq) break;
+    }
+    *Qcount = qcount;
+    return ORC_OK;
+}
+
+/* cone_cut, src/gluonAna/gpManip.f90:59-163, with every optional argument present (as the
+ * C wrapper c_wrapper/FLUE_c.f90:58-71 passes them); Q, QOUT are (NQ, 4) in Fortran order */
+int or_cone_cut(int32_t NQ, const double* Q, const double* D, const double* D4, double angle, double xi,
+                double IRCut, double IRRadius, double* QOUT, double* DOUT, double* D4OUT, int32_t* qcount) {
+    if (NQ < 0) return ORC_EINVAL;
+    int32_t n = 0;
+    for (int32_t i = 0; i < 4 * NQ; ++i) QOUT[i] = 0.0;
+    for (int32_t i = 0; i < NQ; ++i) DOUT[i] = D4OUT[i] = 0.0;
+    for (int32_t k = 0; k < NQ; ++k) {
+        double qh[4];
+        for (int i = 0; i < 4; ++i) qh[i] = Q[k + (int64_t)NQ * i];
+        const double q_norm = sqrt(((qh[0] * qh[0] + qh[1] * qh[1]) + qh[2] * qh[2]) + qh[3] * qh[3]);
+        qh[0] = qh[0] * xi;
+        double r, theta;
+        if (qh[1] == qh[0] && qh[2] == qh[0] && qh[3] == qh[0]) {
+            r = 0.0;
+            theta = 0.0;
+        } else {
+            theta = acos((((0.5 * qh[0] + 0.5 * qh[1]) + 0.5 * qh[2]) + 0.5 * qh[3]) / q_norm);
+            r = q_norm * sin(theta);
+        }
+        const int keep = q_norm <= IRCut ? (r <= IRRadius) : (r <= IRRadius && theta < angle);
+        if (keep) {
+            for (int i = 0; i < 4; ++i) QOUT[n + (int64_t)NQ * i] = Q[k + (int64_t)NQ * i];
+            DOUT[n] = D[k];
+            D4OUT[n] = D4[k];
+            ++n;
+        }
+    }
+    *qcount = n;
+    return ORC_OK;
+}

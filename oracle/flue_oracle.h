@@ -74,6 +74,20 @@ int or_fd_link(or_cplx* data, const int32_t dims[4], int32_t which, double beta,
 /* Wilson flow, Luescher RK3 with Z = -P_TA(U * staple-sum) */
 int or_wilson_flow(or_cplx* data, const int32_t dims[4], double eps, int32_t nsteps, int32_t reunit);
 
+/* Gluon propagator and momentum-space analysis (src/gluonCalc/gluonProp.f90,
+ * src/gluonAna/gpManip.f90).  Uq is the Fourier-transformed field with extents E in the
+ * gauge-field order; D is complex128 over dext in Fortran order.  Where the reference's
+ * result is undefined (U(-q) read past the end for q = 0, qhat(0) for mu = 0, D accumulated
+ * uninitialised, Q_Average's endless loop) these restate a fixed reading, described at each
+ * function; the tests pin that reading by closed forms, not against a reference answer. */
+int or_mom_scalar_d(const or_cplx* Uq, const int32_t E[4], const int32_t dext[4], int32_t mu_start, double xi,
+                    or_cplx* D);
+int or_scalar_gluon_prop(const or_cplx* data, const int32_t dims[4], double xi, double* D);
+int or_q_average(int32_t NQ, const double* q, const double* D, int32_t q_slices, double* DOUT, double* QOUT,
+                 int32_t* Qcount);
+int or_cone_cut(int32_t NQ, const double* Q, const double* D, const double* D4, double angle, double xi,
+                double IRCut, double IRRadius, double* QOUT, double* DOUT, double* D4OUT, int32_t* qcount);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,12 +60,21 @@ def test_host_helpers_match_reference_semantics():
                                   [1, 5, 3, 1])
 
 
-def test_propagator_entries_are_out_of_scope():
+def test_propagator_argument_errors_before_device_work():
+    from flue_lat_amd._lib import FlueError
     import flue_lat_amd as FL
-    with pytest.raises(NotImplementedError):
-        FL.gluonCalc.scalarGluonProp(np.zeros((2, 2, 2, 2, 4, 3, 3), complex), 1.0)
-    with pytest.raises(NotImplementedError):
-        FL.gluonAna.cone_cut(1, 1, None, None, None, 2, 2, 0.0, 1.0, 0.0, 0.0)
+    # D(NS,NS,NS,NS) cannot hold a field with NT > NS
+    with pytest.raises(FlueError) as e:
+        FL.gluonCalc.scalarGluonProp(np.zeros((5, 4, 4, 4, 4, 3, 3), complex), 1.0)
+    assert e.value.status == 5
+    with pytest.raises(FlueError) as e:
+        FL.gluonCalc.calc_mom_space_scalarD(np.zeros((5, 4, 4, 4, 4, 3, 3), complex), -1, 1.0)
+    assert e.value.status == 1
+    with pytest.raises(ValueError):
+        FL.gluonCalc.calc_mom_space_scalarD(np.zeros((5, 4, 4, 3, 4, 3, 3), complex), 1, 1.0)
+    with pytest.raises(FlueError) as e:
+        FL.gluonAna.q_average(3, [1.0, 2.0, 3.0], [0.0, 0.0, 0.0], 0)
+    assert e.value.status == 1
 
 
 def test_argument_errors_before_device_work(L):
