@@ -28,13 +28,68 @@ __device__ __forceinline__ void write_partial(double* partials, uint32_t blk, DD
 
 // sum over sites and planes mu0 <= mu <= mu1, mu < nu <= nu1 of
 // ReTr[(U_mu(x) U_nu(x+mu)) (U_nu(x) U_mu(x+nu))^+]   (src/wloops.F90:947-1006)
+//
+// One thread walks kPlaqLT consecutive time slices of one spatial site.  The spatial links
+// at x + t, which the temporal planes (0, nu) read as neighbours, are kept in registers as
+// the own links of the next slice, so each site reads its own links once and, per slice,
+// 1 temporal link, 3 links at x + t and 9 links of spatial neighbours (against 21 link
+// reads for the planes taken one by one).
+constexpr int kPlaqLT = 4;
+
 template <typename FV>
-__global__ void __launch_bounds__(kBS) plaquette_kernel(Lat L, FV f, int mu0, int mu1, int nu1,
+__global__ void __launch_bounds__(kBS) plaquette_kernel(Lat L, FV f, int mu0, int mu1, int nu1, uint32_t nbs,
                                                         double* __restrict__ partials) {
+    using R = typename FV::real;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = blk * kBS + threadIdx.x;
+    const uint32_t tc = blk / nbs, pb = blk - tc * nbs;
+    const uint32_t p = pb * kBS + threadIdx.x;
     double acc = 0.0;
-    if (s < L.V) acc = plaquette_site(L, f, s, mu0, mu1, nu1);
+    if (p < L.Vs) {
+        const int NT = L.n[0];
+        const int t0 = (int)tc * kPlaqLT, t1 = min(t0 + kPlaqLT, NT);
+        int c[4];
+        site_coords(L, p, c);
+        // forward spatial neighbours within a slice
+        uint32_t sf[4];
+#pragma unroll
+        for (int d = 1; d < 4; ++d) sf[d] = c[d] == L.n[d] - 1 ? p - (uint32_t)(L.n[d] - 1) * L.st[d] : p + L.st[d];
+        const bool tplanes = mu0 == 0;
+        Mat<R> Us[3];
+        uint32_t s = p + L.Vs * (uint32_t)t0;
+        for (int t = t0; t < t1; ++t) {
+            const uint32_t ts = L.Vs * (uint32_t)t;
+            if (!tplanes || t == t0) {
+#pragma unroll
+                for (int nu = 1; nu < 4; ++nu)
+                    if (nu <= nu1) Us[nu - 1] = load_link(f, nu, s);
+            }
+#pragma unroll
+            for (int mu = 1; mu < 3; ++mu) {
+                if (mu < mu0 || mu > mu1) continue;
+#pragma unroll
+                for (int nu = mu + 1; nu < 4; ++nu) {
+                    if (nu > nu1) continue;
+                    const Mat<R> A = mul(Us[mu - 1], load_link(f, nu, sf[mu] + ts));
+                    const Mat<R> B = mul(Us[nu - 1], load_link(f, mu, sf[nu] + ts));
+                    acc += (double)retr_nd(A, B);
+                }
+            }
+            if (tplanes) {
+                const uint32_t sn = t == NT - 1 ? p : s + L.Vs;  // x + t
+                const Mat<R> U0 = load_link(f, 0, s);
+#pragma unroll
+                for (int nu = 1; nu < 4; ++nu) {
+                    if (nu > nu1) continue;
+                    const Mat<R> N = load_link(f, nu, sn);
+                    const Mat<R> A = mul(U0, N);
+                    const Mat<R> B = mul(Us[nu - 1], load_link(f, 0, sf[nu] + ts));
+                    acc += (double)retr_nd(A, B);
+                    Us[nu - 1] = N;
+                }
+            }
+            s += L.Vs;
+        }
+    }
     write_partial(partials, blk, block_sum<kBS>(DD{acc, 0.0}));
 }
 
@@ -172,13 +227,18 @@ PathArg make_path(const int32_t* path, int npath) {
 
 }  // namespace
 
-uint32_t plaquette_blocks(const Lat& L) { return (L.V + kBS - 1) / kBS; }
+// blocks per time chunk (the spatial sites of one slice), times the time chunks
+static uint32_t plaquette_slice_blocks(const Lat& L) { return (L.Vs + kBS - 1) / kBS; }
+
+uint32_t plaquette_blocks(const Lat& L) {
+    return plaquette_slice_blocks(L) * (uint32_t)((L.n[0] + kPlaqLT - 1) / kPlaqLT);
+}
 
 hipError_t launch_plaquette(const Lat& L, const void* U, int is_f32, int mu0, int mu1, int nu1, double* partials,
                             hipStream_t st) {
-    const uint32_t g = plaquette_blocks(L);
+    const uint32_t g = plaquette_blocks(L), nbs = plaquette_slice_blocks(L);
     with_view(L.V, U, is_f32, [&](auto f) {
-        plaquette_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, mu0, mu1, nu1, partials);
+        plaquette_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, mu0, mu1, nu1, nbs, partials);
     });
     return hipGetLastError();
 }
