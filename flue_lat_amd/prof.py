@@ -1,9 +1,10 @@
-"""Profiling driver: runs the magnetic (5-loop F_munu + B^2) measurement a few times
-on a synthetic field and, for calibrating memory counters, one plain device copy of
-known size.  Meant to sit under a profiler:
+"""Profiling driver: runs one device operation (by default the magnetic 5-loop F_munu + B^2
+measurement) a few times on a synthetic field and, for calibrating memory counters, one
+plain device copy of known size.  Meant to sit under a profiler:
 
     rocprofv3 --kernel-trace --stats -d out -- python3 -m flue_lat_amd.prof
     rocprofv3 --pmc FETCH_SIZE -d out -- python3 -m flue_lat_amd.prof --prec fp32
+    rocprofv3 --kernel-trace --stats -d out -- python3 -m flue_lat_amd.prof --op all --copy-mib 0
 """
 import argparse
 import sys
@@ -15,7 +16,8 @@ def main(argv=None):
     ap.add_argument("--prec", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the calibration copy (0 = none)")
-    ap.add_argument("--op", choices=["magnetic", "plaquette", "polyakov", "fmunu"], default="magnetic")
+    ops = ["magnetic", "plaquette", "polyakov", "fmunu", "wloops", "propagator", "wilson_grad", "flow"]
+    ap.add_argument("--op", choices=ops + ["all"], default="magnetic")
     a = ap.parse_args(argv)
 
     import torch
@@ -29,17 +31,31 @@ def main(argv=None):
         fields.append(f64.astype("fp32"))
     if a.prec in ("fp64", "both"):
         fields.append(f64)
-    for g in fields:
-        for _ in range(a.reps):
-            if a.op == "magnetic":
-                v = g.magnetic()
-            elif a.op == "plaquette":
-                v = g.plaquette()[0]
-            elif a.op == "polyakov":
-                v = g.polyakov()[0]
-            else:
-                v = float(g.fmunu(2, 3).abs().sum())
-        print(f"{a.op} {g.precision} {tuple(a.dims)}: {v!r}", flush=True)
+
+    def run(op, g):
+        if op == "magnetic":
+            return g.magnetic()
+        if op == "plaquette":
+            return g.plaquette()[0]
+        if op == "polyakov":
+            return g.polyakov()[0]
+        if op == "fmunu":
+            return float(g.fmunu(2, 3).abs().sum())
+        if op == "wloops":
+            return float(g.wilson_loops(2, 1, 6, 6).sum())
+        if op == "propagator":
+            return float(g.gluon_propagator(mu_start=1).real.sum())
+        if op == "wilson_grad":
+            return float(g.wilson_grad(6.0).data.abs().sum())
+        h = g.clone()
+        h.wilson_flow(0.01, 1, use_graph=False)
+        return h.plaquette()[0]
+
+    for op in (ops if a.op == "all" else [a.op]):
+        for g in fields:
+            for _ in range(a.reps):
+                v = run(op, g)
+            print(f"{op} {g.precision} {tuple(a.dims)}: {v!r}", flush=True)
     if a.copy_mib > 0:
         n = a.copy_mib * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
