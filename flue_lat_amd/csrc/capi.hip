@@ -523,6 +523,8 @@ int flue_scalar_gluon_prop(const int32_t dims[4], const void* U, int32_t prec, d
     if (check_prec(prec)) return FLUE_EINVAL;
     // D(NS,NS,NS,NS) is written at every (t,x,y,z) of the field
     if (dims[2] != dims[1] || dims[3] != dims[1] || dims[0] > dims[1]) return FLUE_EDIM;
+    const uint64_t ns = (uint64_t)dims[1];
+    if (ns * ns * ns * ns > kMaxSites) return FLUE_EDIM;
     return hip_status(launch_scalar_gluon(make_lat(dims), U, prec == FLUE_F32, dims[1], d_D, S(stream)));
 }
 
@@ -1090,7 +1092,9 @@ int calc_mom_space_scalard_c(const int32_t* NS, const int32_t* NT, const flue_c1
 
 int scalargluonprop_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, const double* xi, double* D) {
     if (!NS || !NT || !U || !xi || !D) return FLUE_ENULL;
-    if (*NT > *NS) return FLUE_EDIM;
+    if (*NT > *NS || *NS < 1) return FLUE_EDIM;
+    const uint64_t ns = (uint64_t)*NS;
+    if (ns * ns * ns * ns > kMaxSites) return FLUE_EDIM;
     HostField h;
     int rc;
     if ((rc = load_host_field(NT, NS, NS, NS, U, h))) return rc;

@@ -26,6 +26,10 @@ CASES = [
 ]
 
 
+# position-space propagator case (NT <= NS)
+PROP_CASE = ("prop_4x4", (4, 4, 4, 4), 1005, 0.3)
+
+
 def checksums(U):
     return {"sum_re": float(U.real.sum()), "sum_im": float(U.imag.sum()),
             "sum_abs2": float((np.abs(U) ** 2).sum()),
@@ -48,7 +52,18 @@ def main():
         F = O.loop5_munu(U, 2, 3)
         e["loop5_23_site0"] = [[F[0, 0, 0, 0, a, b].real, F[0, 0, 0, 0, a, b].imag] for a in range(3) for b in range(3)]
         e["wilson_action_beta6"] = O.wilson_action(U, 6.0)
+        # calc_mom_space_scalarD of the field's transform, over the loop range E//2+1
+        Uq = np.fft.fftn(U, axes=(0, 1, 2, 3))
+        for key, (mu_start, xi) in (("gluon_mom_d_mu1", (1, 1.0)), ("gluon_mom_d_mu0_xi15", (0, 1.5))):
+            D = O.mom_scalar_d(Uq, mu_start, xi).real
+            e[key] = {"sum": float(D.sum()), "q0000": float(D[0, 0, 0, 0]), "q1111": float(D[1, 1, 1, 1]),
+                      "qmax": float(D[-1, -1, -1, -1])}
         out[name] = e
+    # scalarGluonProp needs NT <= NS
+    U = O.random_field(PROP_CASE[1], PROP_CASE[2], PROP_CASE[3])
+    D = O.scalar_gluon_prop(U, 1.0)
+    out["scalar_gluon_prop"] = {"dims": list(PROP_CASE[1]), "seed": PROP_CASE[2], "eps": PROP_CASE[3],
+                                "sum": float(D.sum()), "d0000": float(D[0, 0, 0, 0]), "d1231": float(D[1, 2, 3, 1])}
     with open(os.path.join(HERE, "observables.json"), "w") as fh:
         json.dump(out, fh, indent=1)
 

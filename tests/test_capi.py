@@ -75,6 +75,10 @@ def test_propagator_argument_errors_before_device_work():
     with pytest.raises(FlueError) as e:
         FL.gluonAna.q_average(3, [1.0, 2.0, 3.0], [0.0, 0.0, 0.0], 0)
     assert e.value.status == 1
+    # D(NS^4) past 2^31 entries is refused before the field is touched (the pointer is never read)
+    from flue_lat_amd import _lib
+    dims = (ctypes.c_int32 * 4)(1, 256, 256, 256)
+    assert _lib.lib().flue_scalar_gluon_prop(dims, ctypes.c_void_p(64), 0, ctypes.c_void_p(64), None) == 5
 
 
 def test_argument_errors_before_device_work(L):
