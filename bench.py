@@ -50,6 +50,61 @@ def time_steps(fn, steps, warmup, torch):
     return ev0.elapsed_time(ev1) / 1e3 / steps, wall / steps
 
 
+def timed(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
+def cpu_rows(U, threads):
+    """The oracle on the other operations of the path, one pass each over bounded samples
+    (per-site work of every operation is independent of the lattice size)."""
+    import numpy as np
+
+    import oracle as O
+    V = U.shape[0] * U.shape[1] * U.shape[2] * U.shape[3]
+    small = O.random_field((8, 24, 24, 24), SEED, EPS)
+    Vs = 8 * 24 ** 3
+    rows = {}
+    for op, fn, n, lat in (
+            ("plaquette", lambda: O.plaquette(U, 1, 4, 4), V, U.shape),
+            ("polyakov", lambda: O.polyakov(U), V, U.shape),
+            ("fmunu", lambda: O.loop5_munu(U, 2, 3), V, U.shape),
+            ("wilson_loops_6x6", lambda: [O.wmunu(small, 2, 1, R, T) for R in range(1, 7) for T in range(1, 7)],
+             Vs, small.shape),
+            ("propagator", lambda: O.mom_scalar_d(np.fft.fftn(small, axes=(0, 1, 2, 3)), 1, 1.0), Vs, small.shape)):
+        dt = timed(fn)
+        rows[op] = {"gsites": n / dt / 1e9, "seconds": round(dt, 3),
+                    "sample": "x".join(str(d) for d in lat[1:4]) + f"x{lat[0]}"}
+    rows["threads"] = threads
+    return rows
+
+
+def gpu_rows(fields, torch):
+    """Event-timed kernels of the other operations on the benchmark's fields (ms per call,
+    mean of 5 after one warm-up; not part of the headline)."""
+    V = DIMS[0] * DIMS[1] * DIMS[2] * DIMS[3]
+    out = {}
+    for name, g in fields:
+        ops = {
+            "plaquette": lambda: g.plaquette(1, 4, 4, sync=False),
+            "polyakov": lambda: g.polyakov(sync=False),
+            "fmunu": lambda: g.fmunu(2, 3),
+            "wilson_loops_6x6": lambda: g.wilson_loops(2, 1, 6, 6),
+            "propagator": lambda: g.gluon_propagator(mu_start=1),
+            "wilson_grad": lambda: g.wilson_grad(6.0),
+        }
+        r = {}
+        for op, fn in ops.items():
+            t_ev, _ = time_steps(fn, 5, 1, torch)
+            r[op] = {"ms": round(t_ev * 1e3, 4), "gsites": round(V / t_ev / 1e9, 4)}
+        link = 72 if name == "fp32" else 144
+        r["plaquette"]["hbm_frac"] = round(4 * link * V / (r["plaquette"]["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+        r["polyakov"]["hbm_frac"] = round(link * V / (r["polyakov"]["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+        out[name] = r
+    return out
+
+
 def cpu_baseline(nslices=8, min_seconds=12.0):
     """The CPU oracle (FP64, OpenMP) on a bounded sample of the same workload: a 48^3 x
     nslices lattice from the same generator, measured repeatedly until min_seconds of
@@ -70,7 +125,8 @@ def cpu_baseline(nslices=8, min_seconds=12.0):
     sites = reps * nslices * DIMS[1] * DIMS[2] * DIMS[3]
     return {"value": sites / dt / 1e9, "unit": "Gsites/s", "cores": threads, "kind": "port",
             "sample": f"oracle magnetic (FP64, {threads} OpenMP threads) over {reps} passes of a "
-                      f"48^3x{nslices} lattice: {sites} sites in {dt:.2f} s"}
+                      f"48^3x{nslices} lattice: {sites} sites in {dt:.2f} s",
+            "rows": cpu_rows(U, threads)}
 
 
 def hbm_traffic():
@@ -128,6 +184,7 @@ def main():
     rel_B = abs(res["fp32"]["B"] - res["fp64"]["B"]) / abs(res["fp64"]["B"])
 
     traffic = hbm_traffic()
+    rows = gpu_rows((("fp32", f32), ("fp64", f64)), torch)
 
     t32 = res["fp32"]["s_per_step"]
     achieved_tflops = FLOP_PER_SITE * V / t32 / 1e12
@@ -153,6 +210,7 @@ def main():
         "rel_err_B_fp32_vs_fp64": rel_B,
         "B": {"fp32": res["fp32"]["B"], "fp64": res["fp64"]["B"]},
         "observables": extras,
+        "rows": rows,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved_tflops, 3),
