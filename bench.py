@@ -146,8 +146,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: the path runs on a single MI355X (see DESIGN.md for how it shards); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the path runs on a single MI355X, so only --gpus 1 is measured",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
