@@ -244,6 +244,17 @@ class GaugeField:
                   "flue_gluon_prop_mom")
         return D
 
+    def scalar_gluon_propagator(self):
+        """scalarGluonProp (src/gluonCalc/gluonProp.f90:17-110) of this field (NT <= NX = NY =
+        NZ): D(t, x, y, z) as a float64 tensor of shape (NS, NS, NS, NS), 0 for t >= NT."""
+        NS = self.dims[1]
+        buf = torch.empty(NS ** 4, dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_scalar_gluon_prop(self._cdims, _ptr(self.data), self.prec, _ptr(buf), _stream()),
+                  "flue_scalar_gluon_prop")
+        # Fortran order: t fastest
+        return buf.view(NS, NS, NS, NS).permute(3, 2, 1, 0)
+
     def magnetic(self, sync=True, variant=0):
         """B = -(1/V) sum_x sum_i ReTr F_i^2 over the spatial planes (src/wloops.F90:479-544).
         variant selects the kernel form and occupancy target: 1-3 site-major, 4-6

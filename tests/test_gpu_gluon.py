@@ -83,16 +83,12 @@ def test_host_scalar_gluon_prop_matches_oracle(FL, NT, NS):
 
 
 def test_device_scalar_gluon_prop_fp32_field(FL):
-    import ctypes
-    from flue_lat_amd import _lib
     NT, NS = 4, 4
     U = O.random_field((NT, NS, NS, NS), 99, 0.4)
     g = FL.GaugeField.from_complex(U).astype("fp32")
-    D = torch.empty(NS ** 4, dtype=torch.float64, device=g.device)
-    dims = (ctypes.c_int32 * 4)(NT, NS, NS, NS)
-    _lib.check(_lib.lib().flue_scalar_gluon_prop(dims, ctypes.c_void_p(g.data.data_ptr()), _lib.FLUE_F32,
-                                                 ctypes.c_void_p(D.data_ptr()), None), "flue_scalar_gluon_prop")
-    torch.cuda.synchronize()
+    D = g.scalar_gluon_propagator().cpu().numpy()
     # the oracle sees the same FP32-rounded links, so the FP64 bound applies
     want = O.scalar_gluon_prop(g.to_complex().cpu().numpy(), 1.0)
-    close(D.cpu().numpy().reshape((NS,) * 4, order="F"), want, 1e-12)
+    close(D, want, 1e-12)
+    with pytest.raises(FL.FlueError):
+        FL.GaugeField.random((5, 4, 4, 4), 1, 0.3).scalar_gluon_propagator()
