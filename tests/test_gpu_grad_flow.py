@@ -207,3 +207,29 @@ def test_flow_fp32_close_to_fp64(FL):
     g.wilson_flow(0.02, 5, use_graph=False)
     torch.cuda.synchronize()
     assert np.abs(g.to_complex().cpu().numpy() - W).max() < 2e-6
+
+
+def test_flow_stage_entry_composes_to_one_step(FL):
+    # three calls of the single-stage entry (buffers ping-ponged, one shared accumulator)
+    # make one RK3 step, as the oracle's flow does
+    import ctypes
+    from flue_lat_amd import _lib
+    U = O.random_field((4, 5, 4, 3), 407, 0.3)
+    W = O.wilson_flow(U, 0.03, 1)
+    g = gfield(FL, U)
+    bufs = [g.data, torch.empty_like(g.data)]
+    acc = torch.zeros(32 * g.V, dtype=torch.float64, device=g.device)
+    dims = (ctypes.c_int32 * 4)(*g.dims)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for st in range(3):
+        src, dst = bufs[st % 2], bufs[(st + 1) % 2]
+        _lib.check(_lib.lib().flue_flow_stage(dims, ctypes.c_void_p(src.data_ptr()), _lib.FLUE_F64,
+                                              ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(acc.data_ptr()),
+                                              0.03, st, stream), "flue_flow_stage")
+    torch.cuda.synchronize()
+    out = FL.GaugeField(g.dims, "fp64", _data=bufs[1])
+    assert np.abs(out.to_complex().cpu().numpy() - W).max() < 1e-12
+    # Win == Wout is refused
+    assert _lib.lib().flue_flow_stage(dims, ctypes.c_void_p(bufs[0].data_ptr()), _lib.FLUE_F64,
+                                      ctypes.c_void_p(bufs[0].data_ptr()), ctypes.c_void_p(acc.data_ptr()),
+                                      0.03, 0, stream) == 1
