@@ -219,6 +219,9 @@ def main():
             "frac": round(achieved_tflops / PEAK_TFLOPS["fp32"], 4),
             "traffic": traffic,
             "flop_per_site": FLOP_PER_SITE,
+            # the code is built without packed FP32 instructions: their rate is half the peak
+            "peak_unpacked": PEAK_TFLOPS["fp32"] / 2,
+            "frac_of_unpacked": round(achieved_tflops / (PEAK_TFLOPS["fp32"] / 2), 4),
             "hbm": {"achieved": round(achieved_gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(achieved_gbs / PEAK_HBM_GBS, 5), "bytes_per_site": BYTES_PER_SITE["fp32"]},
             "fp64": {"achieved": round(FLOP_PER_SITE * V / res["fp64"]["s_per_step"] / 1e12, 3),
