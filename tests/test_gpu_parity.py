@@ -337,3 +337,30 @@ def test_autotune_picks_a_variant_and_keeps_results(FL, c0):
     assert v in range(1, 7) and ms > 0
     assert g.magnetic() == g.magnetic(variant=v)
     assert abs(g.magnetic() - B0) <= 1e-12 * B0
+
+
+# ------------------------------------------------------------------ degenerate extents
+
+@pytest.mark.parametrize("dims", [(3, 1, 2, 5), (4, 2, 1, 1), (1, 3, 2, 2), (5, 2, 2, 1)])
+def test_extents_of_one_and_two(FL, dims):
+    # every 2- and 3-link step wraps one or more times around such directions
+    print(f"degenerate extents {dims}", flush=True)
+    U = O.random_field(dims, 5151, 0.4)
+    g = gfield(FL, U)
+    for args in ((1, 4, 4), (2, 4, 4), (1, 1, 4)):
+        want, n = O.plaquette(U, *args)
+        got, nP = g.plaquette(*args)
+        assert nP == n and abs(got - want) <= 1e-12 * max(1.0, abs(want)), args
+    for v in range(1, 7):
+        assert abs(g.magnetic(variant=v) - O.magnetic(U)) <= 1e-12 * O.magnetic(U), v
+    for (mu, nu) in ((2, 3), (1, 4)):
+        F = g.fmunu(mu, nu).cpu().numpy()
+        Fo = O.loop5_munu(U, mu, nu)
+        assert np.abs(F - Fo).max() <= 1e-12 * max(1.0, np.abs(Fo).max()), (mu, nu)
+    W = g.wilson_loops(2, 1, 3, 3)
+    for R in range(1, 4):
+        for T in range(1, 4):
+            assert abs(W[R - 1, T - 1] - O.wmunu(U, 2, 1, R, T)) < 1e-12, (R, T)
+    W5 = O.wilson_flow(U, 0.02, 1)
+    g.wilson_flow(0.02, 1, use_graph=False)
+    assert np.abs(g.to_complex().cpu().numpy() - W5).max() < 1e-12
