@@ -53,7 +53,7 @@ enum {
     FLUE_EDIM = 5,     /* lattice extent < 1 or volume too large */
     FLUE_EHIP = 6,     /* HIP runtime error */
     FLUE_ENOMEM = 7,   /* device allocation failed */
-    FLUE_ENOTIMPL = 8, /* outside this library's scope */
+    FLUE_ENOTIMPL = 8, /* outside this library's scope (kept for the ABI; no entry returns it) */
     FLUE_ENODEV = 9,   /* no GPU visible */
     FLUE_EWS = 10      /* workspace too small */
 };
