@@ -1,8 +1,8 @@
 /*
  * flue_c.h -- C ABI of libflue_hip.so, the MI355X implementation of FLUE's
  * gluonic-observable path (plaquette family, Polyakov loop, generic path
- * products, Wilson loops, 5-loop F_munu, magnetic B^2, their gradients and the
- * Wilson flow).
+ * products, Wilson loops, 5-loop F_munu, magnetic B^2, their gradients, the
+ * Wilson flow, and the scalar gluon propagator with its momentum-space analysis).
  *
  * Two groups of entry points:
  *
@@ -18,8 +18,9 @@
  * 2. Device entry points (flue_*) on device-resident fields in the library's
  *    SoA layout: 36 complex planes (direction mu, colour a, b) of V sites, each
  *    site an adjacent (re, im) pair: real index (((mu*3 + a)*3 + b)*V + s)*2 + ri
- *    with s = x + NX*(y + NY*(z + NZ*t)), FP64 or FP32 (72 V reals in all).  They are asynchronous on the given hipStream_t (NULL = default
- *    stream), write results to device memory and never synchronise.  Lattice
+ *    with s = x + NX*(y + NY*(z + NZ*t)), FP64 or FP32 (72 V reals in all).
+ *    They are asynchronous on the given hipStream_t (NULL = default stream),
+ *    write results to device memory and never synchronise.  Lattice
  *    sums are deterministic (fixed reduction order) and use compensated FP64
  *    partial sums; `ws` is a device workspace of at least
  *    flue_workspace_bytes(dims) bytes (NULL = a buffer the library keeps per
