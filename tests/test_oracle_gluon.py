@@ -172,3 +172,22 @@ def test_host_cone_cut_matches_oracle():
         np.testing.assert_array_equal(qo, eq)
         np.testing.assert_array_equal(do, ed)
         np.testing.assert_array_equal(d4o, ed4)
+
+
+def test_host_analysis_empty_and_edge_inputs():
+    import flue_lat_amd as FL
+    from flue_lat_amd._lib import FlueError
+    # no momenta: nothing kept
+    qo, do, d4o, n = FL.gluonAna.cone_cut(0, 1, np.zeros((0, 4)), np.zeros(0), np.zeros(0), 4, 4, 1.0, 1.0, 0.0, 1.0)
+    assert n == 0 and qo.shape == (0, 4)
+    # a single value: maxq = q, one slice; the bin (0, q) excludes q itself
+    dout, qout, cnt = FL.gluonAna.q_average(1, [2.0], [5.0], 1)
+    ed, eq, ec = O.q_average(np.array([2.0]), np.array([5.0]), 1)
+    assert cnt == ec == 1 and np.array_equal(qout, eq) and np.array_equal(dout, ed)
+    # all zero momenta: dq = 0, the loop ends after its first pass
+    dout, qout, cnt = FL.gluonAna.q_average(3, [0.0, 0.0, 0.0], [1.0, 2.0, 3.0], 4)
+    assert cnt == 1 and not qout.any()
+    # a NaN or a negative maximum would keep the reference's loop going: refused
+    for q in ([1.0, float("nan")], [-1.0, -2.0]):
+        with pytest.raises(FlueError):
+            FL.gluonAna.q_average(2, q, [0.0, 0.0], 1)
