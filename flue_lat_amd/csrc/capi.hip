@@ -261,11 +261,14 @@ int q_average_c(const int32_t* NQ, const double* q, const double* D, const int32
     std::fill(QOUT, QOUT + n + 1, 0.0);
     const double maxq = *std::max_element(q, q + n);
     const double dq = maxq / (double)*q_slices;
-    // a negative step never reaches the exit test of the reference's loop
-    if (!(dq >= 0.0)) return FLUE_EINVAL;
+    // the reference's loop ends only if the running bin edge reaches max(q): refuse the
+    // inputs on which it would not (a negative step with one slice, a step of 0 below a
+    // positive maximum)
+    if (dq < 0.0 ? !(dq + dq >= maxq) : (dq == 0.0 && maxq > 0.0)) return FLUE_EINVAL;
     double lo = 0.0;
     int32_t count = 1;
-    for (;;) {
+    for (int64_t it = 0;; ++it) {
+        if (it > (int64_t)*q_slices + 2) return FLUE_EINVAL;
         const double hi = lo + dq;
         double sum = 0.0;
         int32_t k = 0;

@@ -108,8 +108,8 @@ int scalargluonprop_c(const int32_t* NS, const int32_t* NT, const flue_c128* U, 
  * averages q over the open bins (k dq, (k+1) dq), dq = max(q)/q_slices, until the next
  * bin's upper edge reaches max(q); DOUT takes D(1) for every kept bin and Qcount is one
  * more than the number of kept bins, as in the reference.  DOUT and QOUT hold NQ + 1
- * values.  FLUE_EINVAL for NQ < 1, q_slices < 1, a NaN in q, or max(q) < 0 (on which the
- * reference's loop does not end). */
+ * values.  FLUE_EINVAL for NQ < 1, q_slices < 1, a NaN in q, and inputs on which the
+ * reference's loop does not end (max(q) < 0 with one slice). */
 int q_average_c(const int32_t* NQ, const double* q, const double* D, const int32_t* q_slices, double* DOUT,
                 double* QOUT, int32_t* Qcount);
 

@@ -1153,10 +1153,12 @@ int or_q_average(int32_t NQ, const double* q, const double* D, int32_t q_slices,
     for (int32_t i = 1; i < NQ; ++i)
         if (q[i] > maxq) maxq = q[i];
     const double dq = maxq / (double)q_slices;
-    if (!(dq >= 0.0)) return ORC_EINVAL; /* the reference loops without end */
+    /* inputs on which the reference's loop does not end */
+    if (dq < 0.0 ? !(dq + dq >= maxq) : (dq == 0.0 && maxq > 0.0)) return ORC_EINVAL;
     double q_pos = 0.0;
     int32_t qcount = 1;
-    for (;;) {
+    for (int64_t it = 0;; ++it) {
+        if (it > (int64_t)q_slices + 2) return ORC_EINVAL;
         double sum = 0.0;
         int32_t n = 0;
         for (int32_t i = 0; i < NQ; ++i)

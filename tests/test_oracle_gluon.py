@@ -187,7 +187,11 @@ def test_host_analysis_empty_and_edge_inputs():
     # all zero momenta: dq = 0, the loop ends after its first pass
     dout, qout, cnt = FL.gluonAna.q_average(3, [0.0, 0.0, 0.0], [1.0, 2.0, 3.0], 4)
     assert cnt == 1 and not qout.any()
-    # a NaN or a negative maximum would keep the reference's loop going: refused
+    # a NaN, or a negative maximum with one slice, would keep the reference's loop going
     for q in ([1.0, float("nan")], [-1.0, -2.0]):
         with pytest.raises(FlueError):
             FL.gluonAna.q_average(2, q, [0.0, 0.0], 1)
+    # with two slices its first exit test holds: one empty pass
+    dout, qout, cnt = FL.gluonAna.q_average(2, [-1.0, -2.0], [0.0, 0.0], 2)
+    ed, eq, ec = O.q_average(np.array([-1.0, -2.0]), np.array([0.0, 0.0]), 2)
+    assert cnt == ec == 1 and not qout.any()
