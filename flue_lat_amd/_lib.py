@@ -73,7 +73,7 @@ SIGNATURES = {
     "flue_wilson_loops": [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _SZ, _P],
     "flue_wilson_loops_slices": [_P, _P, _I, _I, _I, _I, _I, _P, _P, _SZ, _P],
     "flue_fmunu": [_P, _P, _I, _I, _I, _I, _P, _I, _P, _P],
-    "flue_gluon_prop_mom": [_P, _P, _I, _I, _D, _P, _P, _P, _P],
+    "flue_gluon_prop_mom": [_P, _P, _I, _P, _I, _D, _P, _P, _P, _P],
     "flue_scalar_gluon_prop": [_P, _P, _I, _P, _P],
     "flue_magnetic": [_P, _P, _I, _P, _P, _SZ, _P],
     "flue_magnetic_ex": [_P, _P, _I, _I, _P, _P, _SZ, _P],

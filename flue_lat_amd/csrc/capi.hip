@@ -507,16 +507,28 @@ int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, i
                                    layout == FLUE_OUT_SOA, strides, S(stream)));
 }
 
-int flue_gluon_prop_mom(const int32_t dims[4], const void* Uq, int32_t prec, int32_t mu_start, double xi,
-                        const int32_t dext[4], void* d_D, const int64_t dstrides[4], void* stream) {
+int flue_gluon_prop_mom(const int32_t dims[4], const void* Uq, int32_t prec, const int64_t qstrides[4],
+                        int32_t mu_start, double xi, const int32_t dext[4], void* d_D, const int64_t dstrides[4],
+                        void* stream) {
     int rc = check_dims(dims);
     if (rc) return rc;
     if (!Uq || !d_D || !dext || !dstrides) return FLUE_ENULL;
     if (check_prec(prec) || mu_start < 0) return FLUE_EINVAL;
     for (int i = 0; i < 4; ++i)
         if (dext[i] < 1 || dext[i] > dims[i] / 2 + 1) return FLUE_EINVAL;
-    return hip_status(launch_mom_scalar(make_lat(dims), Uq, prec == FLUE_F32, mu_start, xi, dext, dstrides,
-                                        (double*)d_D, S(stream)));
+    Lat Lq = make_lat(dims);
+    if (qstrides) {
+        // any strides that keep every momentum inside its plane of V elements
+        int64_t last = 0;
+        for (int i = 0; i < 4; ++i) {
+            if (qstrides[i] < 1) return FLUE_EINVAL;
+            last += (int64_t)(dims[i] - 1) * qstrides[i];
+        }
+        if (last >= (int64_t)Lq.V) return FLUE_EINVAL;
+        for (int i = 0; i < 4; ++i) Lq.st[i] = (uint32_t)qstrides[i];
+    }
+    return hip_status(launch_mom_scalar(Lq, Uq, prec == FLUE_F32, mu_start, xi, dext, dstrides, (double*)d_D,
+                                        S(stream)));
 }
 
 int flue_scalar_gluon_prop(const int32_t dims[4], const void* U, int32_t prec, double* d_D, void* stream) {
@@ -1088,7 +1100,8 @@ int calc_mom_space_scalard_c(const int32_t* NS, const int32_t* NT, const flue_c1
     const size_t n = (size_t)dst[3] * dext[3];
     DevBuf d;
     FLUE_TRY(hipMalloc(&d.p, n * sizeof(flue_c128)));
-    if ((rc = flue_gluon_prop_mom(h.dims, h.soa.p, FLUE_F64, *mu_start, *xi, dext, d.p, dst, nullptr))) return rc;
+    if ((rc = flue_gluon_prop_mom(h.dims, h.soa.p, FLUE_F64, nullptr, *mu_start, *xi, dext, d.p, dst, nullptr)))
+        return rc;
     FLUE_TRY(hipMemcpy(D, d.p, n * sizeof(flue_c128), hipMemcpyDeviceToHost));
     return FLUE_OK;
 }

@@ -223,24 +223,28 @@ class GaugeField:
 
     def momentum_field(self):
         """U_mu(q) = sum_x exp(-i q.x) U_mu(x) for every link element (numpy.fft.fftn's
-        convention), as a complex tensor (36, NT, NZ, NY, NX) laid out like the field with the
-        momenta in place of the sites."""
+        convention), as a complex tensor of shape (36, NT, NZ, NY, NX): plane (mu, a, b) first,
+        then the momenta in the site order's axes (its strides within a plane may be permuted,
+        as the transform leaves them)."""
         NT, NX, NY, NZ = self.dims
         c = torch.view_as_complex(self.data.view(36, NT, NZ, NY, NX, 2))
-        # a multi-axis transform may hand back permuted strides: the kernel reads it dense
-        return torch.fft.fftn(c, dim=(1, 2, 3, 4)).contiguous()
+        return torch.fft.fftn(c, dim=(1, 2, 3, 4))
 
     def gluon_propagator(self, mu_start=1, xi=1.0):
         """calc_mom_space_scalarD (src/gluonCalc/gluonProp.f90:112-250) of this field: the
-        4-D transform on the device, then the per-momentum colour algebra in one kernel.
-        Returns D as a complex128 tensor of shape (NT//2+1, NX//2+1, NY//2+1, NZ//2+1), the
-        reference's loop range over q."""
+        4-D transform on the device, then the per-momentum colour algebra in one kernel, which
+        reads the transform in the layout it comes in.  Returns D as a complex128 tensor of
+        shape (NT//2+1, NX//2+1, NY//2+1, NZ//2+1), the reference's loop range over q."""
         Uq = self.momentum_field()
+        if Uq.stride(0) != self.V:
+            Uq = Uq.contiguous()
+        st = Uq.stride()
+        qstrides = _i64((st[1], st[4], st[3], st[2]))  # (t, x, y, z)
         dext = tuple(d // 2 + 1 for d in self.dims)
         D = torch.empty(dext, dtype=torch.complex128, device=self.device)
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_gluon_prop_mom(self._cdims, _ptr(Uq), self.prec, int(mu_start), float(xi),
-                                                 _i32(*dext), _ptr(D), _i64(D.stride()), _stream()),
+            check(_lib.lib().flue_gluon_prop_mom(self._cdims, _ptr(Uq), self.prec, qstrides, int(mu_start),
+                                                 float(xi), _i32(*dext), _ptr(D), _i64(D.stride()), _stream()),
                   "flue_gluon_prop_mom")
         return D
 

@@ -231,14 +231,18 @@ int flue_fmunu(const int32_t dims[4], const void* U, int32_t prec, int32_t mu, i
                void* out, int32_t layout, const int64_t strides[6], void* stream);
 
 /* calc_mom_space_scalarD (src/gluonCalc/gluonProp.f90:112-250) on a Fourier-transformed
- * field Uq held in the SoA layout over its extents dims (the momenta q take the place of
- * the sites): d_D (complex128) at element strides dstrides for (t, x, y, z) over
- * 0 <= q < dext, dext[i] <= dims[i]/2 + 1 (the reference's loop range).  U(-q) is read at
+ * field Uq held like a SoA field over its extents dims (36 complex planes of V elements,
+ * the momenta q in place of the sites): within a plane, momentum q sits at
+ * sum_i q_i qstrides[i] (complex elements, order t, x, y, z; NULL = the site order
+ * x + NX(y + NY(z + NZ t)); the strides must keep every q inside the plane).  d_D
+ * (complex128) at element strides dstrides for (t, x, y, z) over 0 <= q < dext,
+ * dext[i] <= dims[i]/2 + 1 (the reference's loop range).  U(-q) is read at
  * (dims - q) mod dims; the phase of direction mu >= 1 uses the momentum component of
  * direction mu - 1 as the reference's qhat(mu) does, and that of mu = 0 the t component.
  * mu_start >= 0; directions mu_start..3 are summed. */
-int flue_gluon_prop_mom(const int32_t dims[4], const void* Uq, int32_t prec, int32_t mu_start, double xi,
-                        const int32_t dext[4], void* d_D, const int64_t dstrides[4], void* stream);
+int flue_gluon_prop_mom(const int32_t dims[4], const void* Uq, int32_t prec, const int64_t qstrides[4],
+                        int32_t mu_start, double xi, const int32_t dext[4], void* d_D, const int64_t dstrides[4],
+                        void* stream);
 
 /* scalarGluonProp (src/gluonCalc/gluonProp.f90:17-110) on a position-space field with
  * NT <= NX = NY = NZ = NS: d_D holds NS^4 doubles, D(t,x,y,z) in Fortran order, 0 for
