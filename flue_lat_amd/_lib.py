@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # FLUE_HIP_LIB may point at another build of the same library (e.g. a tuning variant)
 LIB_PATH = os.environ.get("FLUE_HIP_LIB") or os.path.join(HERE, "libflue_hip.so")
 
-FLUE_F64, FLUE_F32 = 0, 1
+FLUE_F64, FLUE_F32, FLUE_F64_SU3 = 0, 1, 2
 FLUE_C128, FLUE_C64 = 0, 1
 FLUE_OUT_STRIDED, FLUE_OUT_SOA = 0, 1
 

@@ -48,7 +48,7 @@ int check_dims(const int32_t* d) {
     return FLUE_OK;
 }
 
-int check_prec(int32_t p) { return (p == FLUE_F64 || p == FLUE_F32) ? FLUE_OK : FLUE_EINVAL; }
+int check_prec(int32_t p) { return (p == FLUE_F64 || p == FLUE_F32 || p == FLUE_F64_SU3) ? FLUE_OK : FLUE_EINVAL; }
 
 int check_path(const int32_t* path, int32_t npath) {
     if (npath < 0 || npath > 64) return FLUE_EPATH;
@@ -561,7 +561,7 @@ int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t
     if (variant < 0 || variant > kMagneticVariants) return FLUE_EINVAL;
     if (variant == 0) variant = tuned_variant(FLUE_OP_MAGNETIC, dims, prec);
     uint32_t np = 0;
-    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, (double*)w, S(stream), variant, &np));
+    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, prec == FLUE_F64_SU3, (double*)w, S(stream), variant, &np));
     FLUE_TRY(launch_finalize((double*)w, np, d_B, 1.0 / (double)L.V, S(stream)));
     return FLUE_OK;
 }
@@ -706,7 +706,7 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
     if (!U) return FLUE_ENULL;
     if (op != FLUE_OP_MAGNETIC || check_prec(prec) || reps < 1) return FLUE_EINVAL;
     const Lat L = make_lat(dims);
-    const int is_f32 = prec == FLUE_F32;
+    const int is_f32 = prec == FLUE_F32, su3 = prec == FLUE_F64_SU3;
     if (!fits_buffer(L.V, is_f32)) {
         // only one variant exists for fields beyond the buffer-resource range
         if (best) *best = magnetic_default_variant(is_f32);
@@ -727,10 +727,10 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
     float bms = 0.f;
     hipError_t e = hipSuccess;
     for (int v = 1; v <= kMagneticVariants && e == hipSuccess; ++v) {
-        e = launch_magnetic(L, U, is_f32, (double*)w, st, v, nullptr);  // warm-up
+        e = launch_magnetic(L, U, is_f32, su3, (double*)w, st, v, nullptr);  // warm-up
         if (e == hipSuccess) e = hipEventRecord(e0, st);
         for (int r = 0; r < reps && e == hipSuccess; ++r)
-            e = launch_magnetic(L, U, is_f32, (double*)w, st, v, nullptr);
+            e = launch_magnetic(L, U, is_f32, su3, (double*)w, st, v, nullptr);
         if (e == hipSuccess) e = hipEventRecord(e1, st);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;

@@ -60,8 +60,8 @@ constexpr uint32_t kFinalChunk = 2048;
 
 // number of magnetic kernel variants (forms x occupancy targets) the autotuner chooses from
 constexpr int kMagneticVariants = 6;
-hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st, int variant,
-                           uint32_t* npart);
+hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, double* partials, hipStream_t st,
+                           int variant, uint32_t* npart);
 
 // gradients and flow (k_grad.hip, k_b2grad.hip); grad / Wout / Abuf are SoA in the field
 // precision, acc is FP64 (8 su(3) coordinates x 4 directions x V)

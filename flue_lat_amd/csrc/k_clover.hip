@@ -60,14 +60,14 @@ __device__ __forceinline__ int plane_nu(int pl) { return pl == 0 ? 1 : (pl == 1 
 
 // B = -(1/V) sum_x sum_i ReTr(F_i F_i) over the planes (y,x), (x,z), (z,y)
 // (src/wloops.F90:502-506, 530-543); each block writes one compensated partial sum.
-template <typename FV, int WPE, bool PLANE_MAJOR>
+template <typename FV, int WPE, bool PLANE_MAJOR, bool C2 = sizeof(typename FV::real) == 4>
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ partials) {
-    // FP32: link chains carry two rows (SU(3) links; row 2 is rebuilt where a leaf closes).
-    // FP64 keeps full products, so it makes no assumption on the links.  Consecutive
-    // quadrants share their lines where the registers allow it: always in FP32, at one
-    // wave per SIMD in FP64.
-    constexpr bool kC2 = sizeof(typename FV::real) == 4;
+    // C2 (FP32 always; FP64 for links declared SU(3)): link chains carry two rows and row 2
+    // is rebuilt where a leaf closes.  Plain FP64 keeps full products, so it makes no
+    // assumption on the links.  Consecutive quadrants share their lines where the registers
+    // allow it: always with C2, at one wave per SIMD otherwise.
+    constexpr bool kC2 = C2;
     constexpr bool kShare = kC2 || WPE == 1;
     // the plane comes from blockIdx.y so that it, and every buffer descriptor derived
     // from it, stays in scalar registers (a division of blockIdx.x would not)
@@ -167,9 +167,11 @@ int magnetic_default_variant(int is_f32) { return is_f32 ? 5 : 4; }
 
 // variant 1..3: site-major with an occupancy target of 1..3 waves per SIMD;
 // variant 4..6: plane-major with 1..3 waves per SIMD; 0 picks the precision's default.
+// su3 (FP64 only) selects two-row chains for the plane-major forms at 1 and 2 waves per
+// SIMD; the other forms, and fields beyond the buffer-resource range, keep full products.
 // *npart receives the number of partial sums written (nb or 3 nb).
-hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st, int variant,
-                           uint32_t* npart) {
+hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, double* partials, hipStream_t st,
+                           int variant, uint32_t* npart) {
     const uint32_t nb = magnetic_blocks(L, is_f32) / 3;
     if (variant == 0) variant = magnetic_default_variant(is_f32);
     if (variant < 1 || variant > kMagneticVariants) return hipErrorInvalidValue;
@@ -182,6 +184,11 @@ hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, double* part
             // fields beyond the buffer-resource range: one form only
             magnetic_kernel<F, sizeof(R) == 4 ? 2 : 1, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials);
             variant = 1;
+        } else if (sizeof(R) == 8 && su3 && (variant == 4 || variant == 5)) {
+            if (variant == 4)
+                magnetic_kernel<F, 1, true, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials);
+            else
+                magnetic_kernel<F, 2, true, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials);
         } else {
             switch (variant) {
                 case 1: magnetic_kernel<F, 1, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials); break;

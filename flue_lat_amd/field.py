@@ -14,10 +14,14 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import FLUE_C64, FLUE_C128, FLUE_F32, FLUE_F64, FLUE_OUT_SOA, FLUE_OUT_STRIDED, check
+from ._lib import FLUE_C64, FLUE_C128, FLUE_F32, FLUE_F64, FLUE_F64_SU3, FLUE_OUT_SOA, FLUE_OUT_STRIDED, check
 
+# "fp64-su3": FP64 links the caller declares SU(3); the magnetic measurement may then carry
+# two rows of each product, as FP32 always does
 _PREC = {"fp64": FLUE_F64, "f64": FLUE_F64, "double": FLUE_F64, torch.float64: FLUE_F64,
-         "fp32": FLUE_F32, "f32": FLUE_F32, "float": FLUE_F32, torch.float32: FLUE_F32}
+         "fp32": FLUE_F32, "f32": FLUE_F32, "float": FLUE_F32, torch.float32: FLUE_F32,
+         "fp64-su3": FLUE_F64_SU3}
+_PREC_NAME = {FLUE_F64: "fp64", FLUE_F32: "fp32", FLUE_F64_SU3: "fp64-su3"}
 
 
 def _prec(p):
@@ -82,7 +86,7 @@ class GaugeField:
     # ------------------------------------------------------------ construction
     @property
     def precision(self):
-        return "fp32" if self.prec == FLUE_F32 else "fp64"
+        return _PREC_NAME[self.prec]
 
     def workspace(self):
         if self._ws is None:
@@ -132,7 +136,7 @@ class GaugeField:
 
     def astype(self, precision):
         p = _prec(precision)
-        g = GaugeField(self.dims, "fp32" if p == FLUE_F32 else "fp64", self.device)
+        g = GaugeField(self.dims, _PREC_NAME[p], self.device)
         with torch.cuda.device(self.device):
             check(_lib.lib().flue_convert_field(self._cdims, _ptr(self.data), self.prec, _ptr(g.data), g.prec,
                                                 _stream()), "flue_convert_field")

@@ -62,8 +62,11 @@ enum {
 /* link precision of a SoA field.  FLUE_F64 makes no assumption on the links.  FLUE_F32
  * fields must hold SU(3) links (unitary, det 1, to FP32 rounding): the clover kernels
  * carry two rows of each link product and rebuild the third as conj(row0 x row1).
- * Sums and per-site traces are FP64 in both. */
-enum { FLUE_F64 = 0, FLUE_F32 = 1 };
+ * Sums and per-site traces are FP64 in both.  FLUE_F64_SU3 is an FP64 field whose links
+ * the caller declares SU(3) (to FP64 rounding): stored and computed as FLUE_F64, except
+ * that the magnetic measurement (flue_magnetic, variants 4 and 5) carries two rows as FP32
+ * does.  No reference counterpart: an opt-in for fields known to be gauge fields. */
+enum { FLUE_F64 = 0, FLUE_F32 = 1, FLUE_F64_SU3 = 2 };
 enum { FLUE_C128 = 0, FLUE_C64 = 1 };       /* element type of a complex array */
 enum { FLUE_OUT_STRIDED = 0, FLUE_OUT_SOA = 1 };
 

@@ -281,6 +281,23 @@ def test_magnetic_fp32(FL, c0):
     assert rel(gfield(FL, c0, "fp32").magnetic(), want) < 1e-6
 
 
+def test_magnetic_fp64_su3(FL, c0, small):
+    # FP64 links declared SU(3): two-row chains in variants 4 and 5, full products elsewhere;
+    # every variant within the FP64 bound of the oracle on gauge fields
+    for U in (c0, small):
+        want = O.magnetic(U)
+        g = gfield(FL, U).astype("fp64-su3")
+        assert g.precision == "fp64-su3"
+        for v in range(1, 7):
+            assert rel(g.magnetic(variant=v), want) < 1e-12, v
+        best, ms = g.autotune("magnetic", reps=2)
+        assert 1 <= best <= 6 and rel(g.magnetic(), want) < 1e-12
+        # everything else treats the field as FP64
+        s, n = g.plaquette()
+        so, no = O.plaquette(U, 1, 4, 4)
+        assert n == no and rel(s, so) < 1e-12
+
+
 @pytest.mark.parametrize("n", [1, 2, 5])
 def test_magnetic_abelian_flux(FL, n):
     dims = (2, 7, 6, 3)
