@@ -70,3 +70,29 @@ def test_unbuffered_flow_step_acts_slice_by_slice(FL):
     finally:
         del big
         torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_benchmark_size_symmetries(FL, prec, tol):
+    # the 48^3x96 benchmark field: B and the plaquette are invariant under a cyclic shift of
+    # the lattice and under exchanging x and y (with the link directions), which reverses the
+    # orientation of every magnetic plane and maps them onto each other
+    dims = (96, 48, 48, 48)
+    g = FL.GaugeField.random(dims, 1003, 0.3, "fp64").astype(prec)
+    B0 = g.magnetic()
+    P0, _ = g.plaquette()
+    NT, NX, NY, NZ = dims
+    v = g.data.view(4, 9, NT, NZ, NY, NX, 2)
+    try:
+        shifted = FL.GaugeField(dims, prec, _data=torch.roll(v, shifts=(1, 5, 2), dims=(2, 3, 5)).reshape(-1))
+        assert abs(shifted.magnetic() - B0) <= tol * B0
+        assert abs(shifted.plaquette()[0] - P0) <= tol * abs(P0)
+        del shifted
+        # x <-> y: sites (t, z, y, x) -> (t, z, x, y), directions (t, x, y, z) -> (t, y, x, z)
+        swapped = v[[0, 2, 1, 3]].transpose(4, 5).contiguous().reshape(-1)
+        sw = FL.GaugeField(dims, prec, _data=swapped)
+        assert abs(sw.magnetic() - B0) <= tol * B0
+        assert abs(sw.plaquette()[0] - P0) <= tol * abs(P0)
+        del sw, swapped
+    finally:
+        torch.cuda.empty_cache()
