@@ -81,18 +81,24 @@ def test_benchmark_size_symmetries(FL, prec, tol):
     g = FL.GaugeField.random(dims, 1003, 0.3, "fp64").astype(prec)
     B0 = g.magnetic()
     P0, _ = g.plaquette()
+    L0, _ = g.polyakov()
+    W0 = g.wilson_loops(2, 1, 6, 6)
     NT, NX, NY, NZ = dims
     v = g.data.view(4, 9, NT, NZ, NY, NX, 2)
     try:
         shifted = FL.GaugeField(dims, prec, _data=torch.roll(v, shifts=(1, 5, 2), dims=(2, 3, 5)).reshape(-1))
         assert abs(shifted.magnetic() - B0) <= tol * B0
         assert abs(shifted.plaquette()[0] - P0) <= tol * abs(P0)
+        assert abs(shifted.polyakov()[0] - L0) <= tol * 1e3 * max(1.0, abs(L0))
+        assert np.abs(shifted.wilson_loops(2, 1, 6, 6) - W0).max() <= tol
         del shifted
         # x <-> y: sites (t, z, y, x) -> (t, z, x, y), directions (t, x, y, z) -> (t, y, x, z)
         swapped = v[[0, 2, 1, 3]].transpose(4, 5).contiguous().reshape(-1)
         sw = FL.GaugeField(dims, prec, _data=swapped)
         assert abs(sw.magnetic() - B0) <= tol * B0
         assert abs(sw.plaquette()[0] - P0) <= tol * abs(P0)
+        # the (x, t) loops of the field are the (y, t) loops of the exchanged one
+        assert np.abs(sw.wilson_loops(3, 1, 6, 6) - W0).max() <= tol
         del sw, swapped
     finally:
         torch.cuda.empty_cache()
