@@ -92,3 +92,22 @@ def test_device_scalar_gluon_prop_fp32_field(FL):
     close(D, want, 1e-12)
     with pytest.raises(FL.FlueError):
         FL.GaugeField.random((5, 4, 4, 4), 1, 0.3).scalar_gluon_propagator()
+
+
+def test_device_propagator_of_time_tiled_field(FL):
+    # a field repeated k times along t transforms to k U(q') at q_t = k q'_t and to 0 at the
+    # other t momenta; with the 1/V of the prefactor, D(k q'_t, q_s) = k D_small(q'_t, q_s)
+    NT, NS, k = 6, 8, 8
+    U = O.random_field((NT, NS, NS, NS), 2468, 0.5)
+    g = FL.GaugeField.from_complex(U)
+    VS = NS ** 3
+    big = FL.GaugeField((NT * k, NS, NS, NS), "fp64", _data=g.data.view(36, NT, 2 * VS).repeat(1, k, 1).reshape(-1))
+    for mu_start in (0, 1):
+        Db = big.gluon_propagator(mu_start=mu_start, xi=1.2).cpu().numpy()
+        Ds = O.mom_scalar_d(np.fft.fftn(U, axes=(0, 1, 2, 3)), mu_start, 1.2)
+        assert Db.shape == (NT * k // 2 + 1, NS // 2 + 1, NS // 2 + 1, NS // 2 + 1)
+        scale = np.abs(Ds).max() * k
+        close(Db[::k], k * Ds, 1e-12)
+        mask = np.ones(Db.shape[0], bool)
+        mask[::k] = False
+        assert np.abs(Db[mask]).max() <= 1e-12 * scale
