@@ -111,3 +111,20 @@ def test_device_propagator_of_time_tiled_field(FL):
         mask = np.ones(Db.shape[0], bool)
         mask[::k] = False
         assert np.abs(Db[mask]).max() <= 1e-12 * scale
+
+
+def test_device_scalar_gluon_prop_closed_form_at_size(FL):
+    # U_mu(x) = diag(e^{ia}, e^{-ia}, 1): D(x) = -(V^3 sin^2 a / 24) sum_mu cos(2 phi_mu) with
+    # phi_mu = c_mu * 2 pi c_mu / N_mu for the 1-based coordinate c (NT = 12 < NS = 16)
+    NT, NS, a = 12, 16, 0.21
+    U = np.zeros((NT, NS, NS, NS, 4, 3, 3), complex)
+    U[...] = np.diag([np.exp(1j * a), np.exp(-1j * a), 1.0])
+    D = FL.GaugeField.from_complex(U).scalar_gluon_propagator().cpu().numpy()
+    V = NT * NS ** 3
+    c = [np.arange(1, n + 1, dtype=float) for n in (NT, NS, NS, NS)]
+    N = (NT, NS, NS, NS)
+    cs = [np.cos(2 * (ci * 2 * np.pi * ci / n)) for ci, n in zip(c, N)]
+    s = cs[0][:, None, None, None] + cs[1][None, :, None, None] + cs[2][None, None, :, None] + cs[3][None, None, None, :]
+    want = np.zeros((NS,) * 4)
+    want[:NT] = -(V ** 3) * np.sin(a) ** 2 / 24.0 * s
+    close(D, want, 1e-12)
