@@ -34,7 +34,7 @@ __device__ __forceinline__ void write_partial(double* partials, uint32_t blk, DD
 // the own links of the next slice, so each site reads its own links once and, per slice,
 // 1 temporal link, 3 links at x + t and 9 links of spatial neighbours (against 21 link
 // reads for the planes taken one by one).
-constexpr int kPlaqLT = 4;
+constexpr int kPlaqLT = 6;
 
 template <typename FV>
 __global__ void __launch_bounds__(kBS) plaquette_kernel(Lat L, FV f, int mu0, int mu1, int nu1, uint32_t nbs,
