@@ -41,9 +41,9 @@ uint32_t magnetic_blocks(const Lat& L, int is_f32);
 int magnetic_default_variant(int is_f32);
 // rectangular Wilson-loop table (k_wloops.hip): R, T up to kMaxLoopRT
 constexpr int kMaxLoopRT = 6;
-// tslice value selecting one table per time slice (blockIdx.y = slice)
+// tslice value selecting one table per time slice
 constexpr int kEverySlice = -2;
-uint32_t wloop_blocks(const Lat& L, int tslice);
+uint32_t wloop_blocks(const Lat& L, int tslice, int mu, int nu);
 size_t wloop_partial_bytes(const Lat& L, int tslice);
 hipError_t launch_wloop_table(const Lat& L, const void* U, int is_f32, int mu, int nu, int Rmax, int Tmax,
                               int tslice, double* partials, double* out, double scale, hipStream_t st);

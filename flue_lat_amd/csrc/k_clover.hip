@@ -109,8 +109,19 @@ __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(sizeof
 fmunu_kernel(Lat L, FV f, int mu, int nu, void* __restrict__ out, int layout, OutStrides os) {
     using R = typename FV::real;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = blk * kBS + threadIdx.x;
-    if (s >= L.V) return;
+    uint32_t s;
+    if (mu == 0 || nu == 0) {
+        // the leaves reach 3 slices either way: blocks take the same sites of consecutive
+        // slices one after another, so the blocks running together share those slices in L2
+        const uint32_t NT = (uint32_t)L.n[0];
+        const uint32_t chunk = blk / NT, t = blk - chunk * NT;
+        const uint32_t idx = chunk * kBS + threadIdx.x;
+        if (idx >= L.Vs) return;
+        s = idx + L.Vs * t;
+    } else {
+        s = blk * kBS + threadIdx.x;
+        if (s >= L.V) return;
+    }
     int c[4];
     site_coords(L, s, c);
     const AH<R> h = plane_ah<CLOVER, sizeof(R) == 4, sizeof(R) == 4>(plane_geom(L, f, s, c[mu], c[nu], mu, nu));
@@ -206,7 +217,8 @@ hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, dou
 
 hipError_t launch_fmunu(const Lat& L, const void* U, int is_f32, int mu, int nu, int clover_only, void* out,
                         int out_layout, const int64_t s6[6], hipStream_t st) {
-    const uint32_t g = (L.V + kBS - 1) / kBS;
+    // planes containing time: every slice's blocks, slice fastest
+    const uint32_t g = (mu == 0 || nu == 0) ? (uint32_t)L.n[0] * ((L.Vs + kBS - 1) / kBS) : (L.V + kBS - 1) / kBS;
     OutStrides os{0, 0, 0, 0, 0, 0};
     if (s6) os = OutStrides{s6[0], s6[1], s6[2], s6[3], s6[4], s6[5]};
     with_view(L.V, U, is_f32, [&](auto f) {

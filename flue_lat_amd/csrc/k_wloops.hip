@@ -8,6 +8,12 @@
 // the Tmax chains CB(T, .) and extends them by one link per R, and runs one CA chain per
 // R: about Rmax*Tmax + 2*(Rmax + Tmax) products per site for the whole table, against
 // 2(R+T)-1 products for each loop walked on its own.
+//
+// A loop reads links up to 6 sites away along mu and nu.  In a plane containing time those
+// are up to 6 time slices apart, so there the blocks are ordered with the time slice
+// fastest: a block covers kBS sites of one slice, consecutive blocks take the same sites
+// of consecutive slices, and the blocks that run together on an XCD share their links in L2
+// (in storage order the L2 of an XCD held no more than a slice, and 9 in 10 reads missed).
 #include "flue_common.hpp"
 #include "flue_launch.hpp"
 
@@ -26,19 +32,45 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Grid: groups x nb logical blocks, one group per time slice for kEverySlice, else one.
+// With time in the plane the slice is the fastest block index (for the whole lattice, nb
+// counts the blocks of every slice, slice fastest); otherwise the block within the group is.
 template <typename FV>
 __global__ void __launch_bounds__(kBS) wloop_table_kernel(Lat L, FV f, int mu, int nu, int Rmax, int Tmax,
-                                                          int tslice, uint32_t nb, double* __restrict__ partials) {
+                                                          int tslice, uint32_t nb, uint32_t groups,
+                                                          double* __restrict__ partials) {
     using R_ = typename FV::real;
     __shared__ double sh[kMaxLoopRT * kMaxLoopRT][kWaves];
-    // tslice >= 0: that time slice; -1: every site; kEverySlice: one slice per blockIdx.y
-    const int ts = tslice == kEverySlice ? (int)blockIdx.y : tslice;
-    const uint32_t blk = xcd_block(blockIdx.x, nb);
-    const uint32_t idx = blk * kBS + threadIdx.x;
-    const uint32_t n = ts >= 0 ? L.Vs : L.V;
-    const bool live = idx < n;
+    const bool tplane = mu == 0 || nu == 0;
+    const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t grp, blk;
+    if (tslice == kEverySlice && tplane) {
+        grp = g % groups;
+        blk = g / groups;
+    } else {
+        grp = g / nb;
+        blk = g - grp * nb;
+    }
+    // tslice >= 0: that time slice; -1: every site; kEverySlice: slice grp
+    const int ts = tslice == kEverySlice ? (int)grp : tslice;
+    bool live;
+    uint32_t s;
+    if (ts >= 0) {
+        const uint32_t idx = blk * kBS + threadIdx.x;
+        live = idx < L.Vs;
+        s = idx + L.Vs * (uint32_t)ts;
+    } else if (tplane) {
+        const uint32_t NT = (uint32_t)L.n[0];
+        const uint32_t chunk = blk / NT, t = blk - chunk * NT;
+        const uint32_t idx = chunk * kBS + threadIdx.x;
+        live = idx < L.Vs;
+        s = idx + L.Vs * t;
+    } else {
+        s = blk * kBS + threadIdx.x;
+        live = s < L.V;
+    }
     // lanes past the end work on site 0 (in bounds) and contribute nothing
-    const uint32_t s = live ? (ts >= 0 ? idx + L.Vs * (uint32_t)ts : idx) : 0u;
+    if (!live) s = 0u;
     int c[4];
     site_coords(L, s, c);
     const int cmu = c[mu], cnu = c[nu];
@@ -93,7 +125,7 @@ __global__ void __launch_bounds__(kBS) wloop_table_kernel(Lat L, FV f, int mu, i
         DD r = {0.0, 0.0};
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) r = dd_add(r, DD{sh[k][w], 0.0});
-        double* p = partials + 2 * (((size_t)k * gridDim.y + blockIdx.y) * nb + blk);
+        double* p = partials + 2 * (((size_t)k * groups + grp) * nb + blk);
         p[0] = r.hi;
         p[1] = r.lo;
     }
@@ -115,21 +147,28 @@ __global__ void __launch_bounds__(kBS) wloop_finalize_kernel(const double* __res
 
 }  // namespace
 
-// blocks per slice group: a single slice (tslice >= 0 or kEverySlice) or the whole lattice
-uint32_t wloop_blocks(const Lat& L, int tslice) { return ((tslice != -1 ? L.Vs : L.V) + kBS - 1) / kBS; }
+// blocks per slice group: a single slice (tslice >= 0 or kEverySlice), or the whole
+// lattice (slice by slice when time is in the plane)
+uint32_t wloop_blocks(const Lat& L, int tslice, int mu, int nu) {
+    const uint32_t per_slice = (L.Vs + kBS - 1) / kBS;
+    if (tslice != -1) return per_slice;
+    return (mu == 0 || nu == 0) ? (uint32_t)L.n[0] * per_slice : (L.V + kBS - 1) / kBS;
+}
 
+// enough for every plane
 size_t wloop_partial_bytes(const Lat& L, int tslice) {
     const size_t groups = tslice == kEverySlice ? (size_t)L.n[0] : 1;
-    return (size_t)kMaxLoopRT * kMaxLoopRT * groups * wloop_blocks(L, tslice) * 2 * sizeof(double);
+    const size_t nb = std::max(wloop_blocks(L, tslice, 0, 1), wloop_blocks(L, tslice, 1, 2));
+    return (size_t)kMaxLoopRT * kMaxLoopRT * groups * nb * 2 * sizeof(double);
 }
 
 hipError_t launch_wloop_table(const Lat& L, const void* U, int is_f32, int mu, int nu, int Rmax, int Tmax,
                               int tslice, double* partials, double* out, double scale, hipStream_t st) {
-    const uint32_t nb = wloop_blocks(L, tslice);
+    const uint32_t nb = wloop_blocks(L, tslice, mu, nu);
     const uint32_t groups = tslice == kEverySlice ? (uint32_t)L.n[0] : 1u;
     with_view(L.V, U, is_f32, [&](auto f) {
         wloop_table_kernel<decltype(f)>
-            <<<dim3(nb, groups), kBS, 0, st>>>(L, f, mu, nu, Rmax, Tmax, tslice, nb, partials);
+            <<<nb * groups, kBS, 0, st>>>(L, f, mu, nu, Rmax, Tmax, tslice, nb, groups, partials);
     });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

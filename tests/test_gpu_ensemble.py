@@ -68,7 +68,7 @@ def test_ensemble_flow_against_oracle(FL):
 
 
 def test_ensemble_c4_shape_fp32(FL):
-    # 32 configs of 16^3 x Nt, Nt from {16,24,32,48,64,128}, eps = 0.4, FP32 links,
+    # ensemble workload: 32 configurations of 16^3 x Nt, Nt from {16,24,32,48,64,128}, eps = 0.4, FP32 links,
     # flow dt = 0.01 with plaquette / B every 10 steps (a short run here; the bench runs 100 steps)
     from flue_lat_amd.ensemble import Ensemble
     rng = np.random.default_rng(4000)

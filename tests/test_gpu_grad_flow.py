@@ -54,7 +54,7 @@ def test_wilson_grad_matches_oracle(FL):
 
 
 def test_wilson_grad_fd_c2(FL):
-    # 16^3x32, eps = 0.3, beta = 6, 64 random links, 8 Gell-Mann directions
+    # gradient check workload: 16^3x32, eps = 0.3, beta = 6, 64 random links, 8 Gell-Mann directions
     dims = (32, 16, 16, 16)
     g = FL.GaugeField.random(dims, 1002, 0.3)
     U = np.asfortranarray(g.to_complex().cpu().numpy())

@@ -29,7 +29,7 @@ def FL():
 
 @pytest.fixture(scope="module")
 def c0():
-    # 8^3x16, eps = 0.25
+    # the small CPU-runnable workload: 8^3x16, eps = 0.25
     return O.random_field((16, 8, 8, 8), 1000, 0.25)
 
 
@@ -189,6 +189,21 @@ def test_wilson_loop_slices(FL, small):
     assert abs(W1[5, 5, 4] - O.wmunutau(small, 5, 1, 4, 6, 6)) < 1e-13
 
 
+def test_wilson_loop_tables_ragged_slices(FL):
+    # 630 sites per slice: two full blocks of 256 and one of 118 per slice, in the
+    # slice-fastest order of the time planes and the storage order of the others
+    U = O.random_field((6, 10, 9, 7), 1017, 0.3)
+    g = gfield(FL, U)
+    for mu, nu in ((2, 1), (1, 4), (3, 4)):
+        W = g.wilson_loops(mu, nu, 6, 6)
+        S = g.wilson_loops_slices(mu, nu, 6, 6)
+        for R, T in ((1, 1), (6, 6), (3, 5), (6, 2)):
+            assert abs(W[R - 1, T - 1] - O.wmunu(U, mu, nu, R, T)) < 1e-13, (mu, nu, R, T)
+            for t in (1, 4, 6):
+                assert abs(S[R - 1, T - 1, t - 1] - O.wmunutau(U, t, mu, nu, R, T)) < 1e-13, (mu, nu, R, T, t)
+        assert np.abs(S.mean(axis=2) - W).max() < 1e-13
+
+
 def test_wilson_loop_table_fp32_and_errors(FL, small):
     W = gfield(FL, small, "fp32").wilson_loops(3, 2, 3, 6)
     for R in range(1, 4):
@@ -246,6 +261,20 @@ def test_loop5_per_site_fp64(FL, small, mu, nu):
     got = gfield(FL, small).fmunu(mu, nu).cpu().numpy()
     scale = np.abs(want).max()
     assert np.abs(got - want).max() < 1e-12 * scale
+
+
+def test_loop5_per_site_ragged_slices(FL):
+    # 630 sites per slice (blocks of 256, 256 and 118 per slice): planes with time take the
+    # blocks slice by slice, the others in storage order; every site and the FP32 form
+    U = O.random_field((6, 10, 9, 7), 1019, 0.3)
+    g, g32 = gfield(FL, U), gfield(FL, U, "fp32")
+    for mu, nu in ((1, 2), (4, 1), (1, 3), (2, 3)):
+        want = O.loop5_munu(U, mu, nu)
+        scale = np.abs(want).max()
+        assert np.abs(g.fmunu(mu, nu).cpu().numpy() - want).max() < 1e-12 * scale, (mu, nu)
+        assert np.abs(g32.fmunu(mu, nu).cpu().numpy() - want).max() < 2e-5 * scale, (mu, nu)
+        clov = O.clover_munu(U, mu, nu)
+        assert np.abs(g.fmunu(mu, nu, clover=True).cpu().numpy() - clov).max() < 1e-12 * np.abs(clov).max()
 
 
 @pytest.mark.parametrize("mu,nu", [(1, 2), (3, 2), (4, 1)])
