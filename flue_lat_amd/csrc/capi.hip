@@ -355,7 +355,7 @@ size_t flue_workspace_bytes(const int32_t dims[4]) {
     const Lat L = make_lat(dims);
     uint32_t nb = plaquette_blocks(L);
     nb = std::max(nb, polyakov_blocks(L));
-    nb = std::max(nb, path_blocks(L, -1));
+    nb = std::max(nb, path_blocks(L, -1, nullptr, 0));
     nb = std::max(nb, magnetic_blocks(L, 0));
     nb = std::max(nb, magnetic_blocks(L, 1));
     return std::max({partial_bytes(nb), wloop_partial_bytes(L, -1) + 256, wloop_partial_bytes(L, kEverySlice) + 256});
@@ -438,7 +438,7 @@ int flue_path_trace(const int32_t dims[4], const void* U, int32_t prec, const in
     if (tau < 0 || tau > dims[0]) return FLUE_EINVAL;
     const Lat L = make_lat(dims);
     const int ts = tau - 1;  // -1 = all slices
-    const uint32_t nb = path_blocks(L, ts);
+    const uint32_t nb = path_blocks(L, ts, path, npath);
     void* w;
     if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
     FLUE_TRY(launch_path_trace(L, U, prec == FLUE_F32, path, npath, ts, (double*)w, S(stream)));

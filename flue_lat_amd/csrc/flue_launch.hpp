@@ -25,7 +25,9 @@ hipError_t launch_plaquette(const Lat& L, const void* U, int is_f32, int mu0, in
                             hipStream_t st);
 uint32_t polyakov_blocks(const Lat& L);
 hipError_t launch_polyakov(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st);
-uint32_t path_blocks(const Lat& L, int tslice);
+// blocks of path_trace over a slice (tslice >= 0) or the whole lattice (-1); with path ==
+// nullptr, enough for any path
+uint32_t path_blocks(const Lat& L, int tslice, const int32_t* path, int npath);
 hipError_t launch_path_trace(const Lat& L, const void* U, int is_f32, const int32_t* path, int npath, int tslice,
                              double* partials, hipStream_t st);
 hipError_t launch_generic_path(const Lat& L, const void* U, int is_f32, const int32_t base[4], const int32_t* path,

@@ -163,14 +163,28 @@ __device__ __forceinline__ Mat<typename FV::real> walk_path(const Lat& L, const 
 }
 
 template <typename FV>
-__global__ void __launch_bounds__(kBS) path_trace_kernel(Lat L, FV f, PathArg p, int tslice,
+__global__ void __launch_bounds__(kBS) path_trace_kernel(Lat L, FV f, PathArg p, int tslice, int by_slice,
                                                          double* __restrict__ partials) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t idx = blk * kBS + threadIdx.x;
-    const uint32_t n = tslice >= 0 ? L.Vs : L.V;
+    uint32_t s;
+    bool live;
+    if (tslice >= 0) {
+        const uint32_t idx = blk * kBS + threadIdx.x;
+        live = idx < L.Vs;
+        s = idx + L.Vs * (uint32_t)tslice;
+    } else if (by_slice) {
+        // the same sites of consecutive slices in consecutive blocks
+        const uint32_t NT = (uint32_t)L.n[0];
+        const uint32_t chunk = blk / NT, t = blk - chunk * NT;
+        const uint32_t idx = chunk * kBS + threadIdx.x;
+        live = idx < L.Vs;
+        s = idx + L.Vs * t;
+    } else {
+        s = blk * kBS + threadIdx.x;
+        live = s < L.V;
+    }
     double acc = 0.0;
-    if (idx < n) {
-        const uint32_t s = tslice >= 0 ? idx + L.Vs * (uint32_t)tslice : idx;
+    if (live) {
         int c[4];
         site_coords(L, s, c);
         acc = (double)retr(walk_path(L, f, p, s, c));
@@ -251,15 +265,32 @@ hipError_t launch_polyakov(const Lat& L, const void* U, int is_f32, double* part
     return hipGetLastError();
 }
 
-uint32_t path_blocks(const Lat& L, int tslice) { return ((tslice >= 0 ? L.Vs : L.V) + kBS - 1) / kBS; }
+// Over the whole lattice, a path with a time step takes its blocks slice-fastest: the
+// blocks in flight then cover the same sites of neighbouring slices and share the links the
+// path reaches there (in storage order one slice lies hundreds of blocks on).
+static bool path_has_time(const int32_t* path, int npath) {
+    for (int i = 0; i < npath; ++i)
+        if (path[i] == 1 || path[i] == -1) return true;
+    return false;
+}
+
+uint32_t path_blocks(const Lat& L, int tslice, const int32_t* path, int npath) {
+    const uint32_t per_slice = (L.Vs + kBS - 1) / kBS;
+    if (tslice >= 0) return per_slice;
+    const uint32_t plain = (L.V + kBS - 1) / kBS, by_slice = (uint32_t)L.n[0] * per_slice;
+    if (!path) return std::max(plain, by_slice);
+    return path_has_time(path, npath) ? by_slice : plain;
+}
 
 hipError_t launch_path_trace(const Lat& L, const void* U, int is_f32, const int32_t* path, int npath, int tslice,
                              double* partials, hipStream_t st) {
     if (npath < 1 || npath > kMaxPath) return hipErrorInvalidValue;
     const PathArg p = make_path(path, npath);
-    const uint32_t g = path_blocks(L, tslice);
-    with_view(L.V, U, is_f32,
-              [&](auto f) { path_trace_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, p, tslice, partials); });
+    const uint32_t g = path_blocks(L, tslice, path, npath);
+    const int by_slice = tslice < 0 && path_has_time(path, npath);
+    with_view(L.V, U, is_f32, [&](auto f) {
+        path_trace_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, p, tslice, by_slice, partials);
+    });
     return hipGetLastError();
 }
 

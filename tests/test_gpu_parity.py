@@ -204,6 +204,23 @@ def test_wilson_loop_tables_ragged_slices(FL):
         assert np.abs(S.mean(axis=2) - W).max() < 1e-13
 
 
+def test_path_trace_ragged_slices_brute_force(FL):
+    # whole-lattice and one-slice sums of ReTr genericPath against the path walked at every
+    # site; paths with a time step take the slice-fastest block order (630 sites per slice)
+    U = O.random_field((6, 10, 9, 7), 1021, 0.3)
+    g = gfield(FL, U)
+    NT, NX, NY, NZ = U.shape[:4]
+    for path in ([2, 1, 3, -2, -1, -3], [1, 1, 1, 1, 1, 1, 1], [2, 3, -2, -3, 4, -4]):
+        per_t = np.zeros(NT)
+        for t in range(NT):
+            for x in range(NX):
+                for y in range(NY):
+                    for z in range(NZ):
+                        per_t[t] += np.trace(O.generic_path(U, [t + 1, x + 1, y + 1, z + 1], path)).real
+        assert abs(g.path_trace(path) - per_t.sum()) < 1e-12 * U[..., 0, 0, 0].size, path
+        assert abs(g.path_trace(path, tau=4) - per_t[3]) < 1e-12 * NX * NY * NZ, path
+
+
 def test_wilson_loop_table_fp32_and_errors(FL, small):
     W = gfield(FL, small, "fp32").wilson_loops(3, 2, 3, 6)
     for R in range(1, 4):
