@@ -422,6 +422,18 @@ __device__ __forceinline__ uint32_t step_bwd(const Lat& L, uint32_t s, int* c, i
     return s - L.stride(mu);
 }
 
+// Offset within a time slice of its idx-th site when the slice is taken x fastest, then
+// direction d, then the remaining spatial direction.  For d = 1 or 2 that is storage order;
+// for d = 3 (z) consecutive sites then span a few rows along z, so kernels whose sites reach
+// several steps along z find those sites in the same block or the next one.
+__device__ __forceinline__ uint32_t slice_site_along(const Lat& L, uint32_t idx, int d) {
+    if (d != 3) return idx;
+    const uint32_t NX = (uint32_t)L.n[1], NZ = (uint32_t)L.n[3];
+    const uint32_t x = idx % NX, r = idx / NX;
+    const uint32_t z = r % NZ, y = r / NZ;
+    return x + NX * (y + (uint32_t)L.n[2] * z);
+}
+
 // Hardware hands consecutive workgroups to the 8 XCDs in turn; give each XCD a
 // contiguous range of logical blocks so that its private L2 sees neighbouring sites.
 __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {

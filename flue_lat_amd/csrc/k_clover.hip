@@ -117,7 +117,7 @@ fmunu_kernel(Lat L, FV f, int mu, int nu, void* __restrict__ out, int layout, Ou
         const uint32_t chunk = blk / NT, t = blk - chunk * NT;
         const uint32_t idx = chunk * kBS + threadIdx.x;
         if (idx >= L.Vs) return;
-        s = idx + L.Vs * t;
+        s = slice_site_along(L, idx, mu + nu) + L.Vs * t;  // mu + nu: the spatial direction
     } else {
         s = blk * kBS + threadIdx.x;
         if (s >= L.V) return;

@@ -53,18 +53,20 @@ __global__ void __launch_bounds__(kBS) wloop_table_kernel(Lat L, FV f, int mu, i
     }
     // tslice >= 0: that time slice; -1: every site; kEverySlice: slice grp
     const int ts = tslice == kEverySlice ? (int)grp : tslice;
+    // in a time plane, the slice is taken along the plane's spatial direction
+    const int d = tplane ? mu + nu : 1;
     bool live;
     uint32_t s;
     if (ts >= 0) {
         const uint32_t idx = blk * kBS + threadIdx.x;
         live = idx < L.Vs;
-        s = idx + L.Vs * (uint32_t)ts;
+        s = (live ? slice_site_along(L, idx, d) : 0u) + L.Vs * (uint32_t)ts;
     } else if (tplane) {
         const uint32_t NT = (uint32_t)L.n[0];
         const uint32_t chunk = blk / NT, t = blk - chunk * NT;
         const uint32_t idx = chunk * kBS + threadIdx.x;
         live = idx < L.Vs;
-        s = idx + L.Vs * t;
+        s = (live ? slice_site_along(L, idx, d) : 0u) + L.Vs * t;
     } else {
         s = blk * kBS + threadIdx.x;
         live = s < L.V;
