@@ -121,6 +121,10 @@ fmunu_kernel(Lat L, FV f, int mu, int nu, void* __restrict__ out, int layout, Ou
     } else {
         s = blk * kBS + threadIdx.x;
         if (s >= L.V) return;
+        if (mu + nu == 4) {  // plane (x, z): each slice taken along z
+            const uint32_t t = s / L.Vs;
+            s = slice_site_along(L, s - t * L.Vs, 3) + L.Vs * t;
+        }
     }
     int c[4];
     site_coords(L, s, c);

@@ -53,8 +53,9 @@ __global__ void __launch_bounds__(kBS) wloop_table_kernel(Lat L, FV f, int mu, i
     }
     // tslice >= 0: that time slice; -1: every site; kEverySlice: slice grp
     const int ts = tslice == kEverySlice ? (int)grp : tslice;
-    // in a time plane, the slice is taken along the plane's spatial direction
-    const int d = tplane ? mu + nu : 1;
+    // in a time plane the slice is taken along the plane's spatial direction; in the (x, z)
+    // plane along z
+    const int d = tplane ? mu + nu : (mu + nu == 4 ? 3 : 1);
     bool live;
     uint32_t s;
     if (ts >= 0) {
@@ -68,8 +69,10 @@ __global__ void __launch_bounds__(kBS) wloop_table_kernel(Lat L, FV f, int mu, i
         live = idx < L.Vs;
         s = (live ? slice_site_along(L, idx, d) : 0u) + L.Vs * t;
     } else {
-        s = blk * kBS + threadIdx.x;
-        live = s < L.V;
+        const uint32_t idx = blk * kBS + threadIdx.x;
+        live = idx < L.V;
+        const uint32_t t = idx / L.Vs;
+        s = d == 1 ? idx : slice_site_along(L, idx - t * L.Vs, d) + L.Vs * t;
     }
     // lanes past the end work on site 0 (in bounds) and contribute nothing
     if (!live) s = 0u;

@@ -191,10 +191,11 @@ def test_wilson_loop_slices(FL, small):
 
 def test_wilson_loop_tables_ragged_slices(FL):
     # 630 sites per slice: two full blocks of 256 and one of 118 per slice, in the
-    # slice-fastest order of the time planes and the storage order of the others
+    # slice-fastest order of the time planes, the storage order of (y, z) and the
+    # along-z slices of (x, z)
     U = O.random_field((6, 10, 9, 7), 1017, 0.3)
     g = gfield(FL, U)
-    for mu, nu in ((2, 1), (1, 4), (3, 4)):
+    for mu, nu in ((2, 1), (1, 4), (3, 4), (2, 4)):
         W = g.wilson_loops(mu, nu, 6, 6)
         S = g.wilson_loops_slices(mu, nu, 6, 6)
         for R, T in ((1, 1), (6, 6), (3, 5), (6, 2)):
@@ -285,7 +286,7 @@ def test_loop5_per_site_ragged_slices(FL):
     # blocks slice by slice, the others in storage order; every site and the FP32 form
     U = O.random_field((6, 10, 9, 7), 1019, 0.3)
     g, g32 = gfield(FL, U), gfield(FL, U, "fp32")
-    for mu, nu in ((1, 2), (4, 1), (1, 3), (2, 3)):
+    for mu, nu in ((1, 2), (4, 1), (1, 3), (2, 3), (4, 2)):
         want = O.loop5_munu(U, mu, nu)
         scale = np.abs(want).max()
         assert np.abs(g.fmunu(mu, nu).cpu().numpy() - want).max() < 1e-12 * scale, (mu, nu)
