@@ -16,7 +16,7 @@ def main(argv=None):
     ap.add_argument("--prec", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the calibration copy (0 = none)")
-    ops = ["magnetic", "plaquette", "polyakov", "fmunu", "wloops", "propagator", "wilson_grad", "flow"]
+    ops = ["magnetic", "plaquette", "polyakov", "fmunu", "wloops", "propagator", "wilson_grad", "flow", "b2_grad"]
     ap.add_argument("--op", choices=ops + ["all"], default="magnetic")
     a = ap.parse_args(argv)
 
@@ -47,6 +47,8 @@ def main(argv=None):
             return float(g.gluon_propagator(mu_start=1).real.sum())
         if op == "wilson_grad":
             return float(g.wilson_grad(6.0).data.abs().sum())
+        if op == "b2_grad":
+            return float(g.magnetic_grad().data.abs().sum())
         h = g.clone()
         h.wilson_flow(0.01, 1, use_graph=False)
         return h.plaquette()[0]
