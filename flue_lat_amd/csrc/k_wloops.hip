@@ -14,6 +14,8 @@
 // fastest: a block covers kBS sites of one slice, consecutive blocks take the same sites
 // of consecutive slices, and the blocks that run together on an XCD share their links in L2
 // (in storage order the L2 of an XCD held no more than a slice, and 9 in 10 reads missed).
+// In the (z, t) and (x, z) planes a slice is also taken with z second-fastest after x, so
+// the loops' z neighbours lie in the same block or the next rather than 9 blocks on.
 #include "flue_common.hpp"
 #include "flue_launch.hpp"
 
