@@ -100,15 +100,13 @@ __device__ __forceinline__ Mat<double> su3_from8(const double v[8]) {
 // acc holds 8 su(3) coordinates per link in FP64 (plane (mu*8+k) of V doubles); the
 // exponential, the product and the reunitarisation (reunit != 0) are FP64.  ca == 0
 // starts a fresh accumulator.
+// the stage for the link U_mu(s) alone (c: the coordinates of s)
 template <typename FV>
-__device__ __forceinline__ void flow_site(const Lat& L, const FV& f, typename FV::real* __restrict__ wout,
-                                          double* __restrict__ acc, uint32_t s, double eps, double ca, double cb,
-                                          int reunit) {
+__device__ __forceinline__ void flow_link(const Lat& L, const FV& f, typename FV::real* __restrict__ wout,
+                                          double* __restrict__ acc, uint32_t s, const int c[4], int mu, double eps,
+                                          double ca, double cb, int reunit) {
     using R = typename FV::real;
-    int c[4];
-    site_coords(L, s, c);
-#pragma unroll 1
-    for (int mu = 0; mu < 4; ++mu) {
+    {
         const Mat<R> U = load_link(f, mu, s);
         const Mat<double> Z = to_double(proj_ta(mul(U, staple_sum(L, f, s, c, mu))));
         double z[8], a[8];
@@ -124,6 +122,16 @@ __device__ __forceinline__ void flow_site(const Lat& L, const FV& f, typename FV
         if (reunit) fix_su3(W);
         store_link(wout, L.V, mu, s, from_double<R>(W));
     }
+}
+
+template <typename FV>
+__device__ __forceinline__ void flow_site(const Lat& L, const FV& f, typename FV::real* __restrict__ wout,
+                                          double* __restrict__ acc, uint32_t s, double eps, double ca, double cb,
+                                          int reunit) {
+    int c[4];
+    site_coords(L, s, c);
+#pragma unroll 1
+    for (int mu = 0; mu < 4; ++mu) flow_link(L, f, wout, acc, s, c, mu, eps, ca, cb, reunit);
 }
 
 // sum over the 6 planes of ReTr of the plaquette at site s (src/wloops.F90:947-1006)

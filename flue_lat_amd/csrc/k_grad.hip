@@ -15,16 +15,20 @@ namespace {
 
 constexpr int kBS = 256;
 
+// One thread per link.  The direction is the fastest block index: the four blocks of the
+// same 256 sites run together and read the same neighbour links, so they share them in L2
+// (a thread walking the four directions of its site read them again long after).
 template <typename FV>
 __global__ void __launch_bounds__(kBS) wilson_grad_kernel(Lat L, FV f, double beta,
                                                           typename FV::real* __restrict__ grad) {
     using R = typename FV::real;
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int mu = (int)(blk & 3u);
+    const uint32_t s = (blk >> 2) * kBS + threadIdx.x;
     if (s >= L.V) return;
     int c[4];
     site_coords(L, s, c);
-#pragma unroll 1
-    for (int mu = 0; mu < 4; ++mu) {
+    {
         const Mat<R> M = mul(load_link(f, mu, s), staple_sum(L, f, s, c, mu));
         Mat<R> G = proj_ta(M);
         const R sc = (R)(beta / 6.0);
@@ -41,14 +45,19 @@ template <typename FV>
 __global__ void __launch_bounds__(kBS) flow_stage_kernel(Lat L, FV f, typename FV::real* __restrict__ wout,
                                                          double* __restrict__ acc, double eps, double ca, double cb,
                                                          int reunit) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int mu = (int)(blk & 3u);
+    const uint32_t s = (blk >> 2) * kBS + threadIdx.x;
     if (s >= L.V) return;
-    flow_site(L, f, wout, acc, s, eps, ca, cb, reunit);
+    int c[4];
+    site_coords(L, s, c);
+    flow_link(L, f, wout, acc, s, c, mu, eps, ca, cb, reunit);
 }
 
 }  // namespace
 
-uint32_t grad_blocks(const Lat& L) { return (L.V + kBS - 1) / kBS; }
+// four blocks (one per direction) per 256 sites
+uint32_t grad_blocks(const Lat& L) { return 4u * ((L.V + kBS - 1) / kBS); }
 
 hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st) {
     const uint32_t g = grad_blocks(L);
