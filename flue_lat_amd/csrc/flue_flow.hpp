@@ -95,12 +95,12 @@ __device__ __forceinline__ Mat<double> su3_from8(const double v[8]) {
     return X;
 }
 
-// One RK3 stage for the four links of site s (Luescher, JHEP 08 (2010) 071, appendix C):
+// One RK3 stage for the link U_mu(s), c the coordinates of s (Luescher, JHEP 08 (2010) 071,
+// appendix C):
 //   Z = -P_TA(W Sigma),  acc <- ca * acc + cb * eps * Z,  W_out = exp(acc) W_in
 // acc holds 8 su(3) coordinates per link in FP64 (plane (mu*8+k) of V doubles); the
 // exponential, the product and the reunitarisation (reunit != 0) are FP64.  ca == 0
-// starts a fresh accumulator.
-// the stage for the link U_mu(s) alone (c: the coordinates of s)
+// starts a fresh accumulator.  Links are independent within a stage: W_in is only read.
 template <typename FV>
 __device__ __forceinline__ void flow_link(const Lat& L, const FV& f, typename FV::real* __restrict__ wout,
                                           double* __restrict__ acc, uint32_t s, const int c[4], int mu, double eps,
@@ -122,16 +122,6 @@ __device__ __forceinline__ void flow_link(const Lat& L, const FV& f, typename FV
         if (reunit) fix_su3(W);
         store_link(wout, L.V, mu, s, from_double<R>(W));
     }
-}
-
-template <typename FV>
-__device__ __forceinline__ void flow_site(const Lat& L, const FV& f, typename FV::real* __restrict__ wout,
-                                          double* __restrict__ acc, uint32_t s, double eps, double ca, double cb,
-                                          int reunit) {
-    int c[4];
-    site_coords(L, s, c);
-#pragma unroll 1
-    for (int mu = 0; mu < 4; ++mu) flow_link(L, f, wout, acc, s, c, mu, eps, ca, cb, reunit);
 }
 
 // sum over the 6 planes of ReTr of the plaquette at site s (src/wloops.F90:947-1006)

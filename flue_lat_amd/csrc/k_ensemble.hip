@@ -56,14 +56,20 @@ __global__ void __launch_bounds__(kBS) ens_flow_kernel(const EnsEntry* __restric
                                                        uint32_t chunk, const R* __restrict__ win, R* __restrict__ wout,
                                                        double* __restrict__ acc, double eps, double ca, double cb,
                                                        int reunit) {
+    // units = (logical block, direction), direction fastest: one thread per link, and the
+    // four units of a block, which read the same neighbour links, run close together
     uint32_t lo, hi;
-    my_range(total, chunk, lo, hi);
-    for (uint32_t lb = lo; lb < hi; ++lb) {
+    my_range(4u * total, chunk, lo, hi);
+    for (uint32_t lu = lo; lu < hi; ++lu) {
+        const uint32_t lb = lu >> 2;
+        const int mu = (int)(lu & 3u);
         const EnsEntry& e = entry(tab, find_cfg(tab, n, lb));
         const uint32_t s = (lb - e.blk0) * kBS + threadIdx.x;
         if (s < e.L.V) {
             const FieldView<R, BUF> f{win + e.off, e.L.V};
-            flow_site(e.L, f, wout + e.off, acc + e.acc_off, s, eps, ca, cb, reunit);
+            int c[4];
+            site_coords(e.L, s, c);
+            flow_link(e.L, f, wout + e.off, acc + e.acc_off, s, c, mu, eps, ca, cb, reunit);
         }
     }
 }
@@ -134,7 +140,7 @@ hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uin
                                  uint32_t grid, hipStream_t st) {
     with_real(is_f32, fits_buffer(maxV, is_f32), [&](auto r, auto buf) {
         using R = decltype(r);
-        ens_flow_kernel<R, decltype(buf)::value><<<grid, kBS, 0, st>>>(tab, n, total, (total + grid - 1) / grid,
+        ens_flow_kernel<R, decltype(buf)::value><<<grid, kBS, 0, st>>>(tab, n, total, (4u * total + grid - 1) / grid,
                                                                       (const R*)win, (R*)wout, acc, eps, ca, cb,
                                                                       reunit);
     });
