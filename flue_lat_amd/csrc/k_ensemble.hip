@@ -5,7 +5,8 @@
 // configuration owns a contiguous range of logical blocks (256 sites each); a persistent
 // grid walks the logical blocks in contiguous chunks per workgroup (chunks of one XCD
 // adjacent), finds the configuration of a logical block by binary search in a table
-// kept in device memory, and runs the same per-site code as the single-field kernels.
+// kept in device memory, and runs the same per-site (flow: per-link) code as the
+// single-field kernels.
 // Sums are reduced per logical block and then per configuration in a fixed order.
 #include "flue_clover.hpp"
 #include "flue_flow.hpp"
