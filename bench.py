@@ -9,7 +9,12 @@ and rounded to FP32 for the FP32 path.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
-Prints one JSON line.  Only one GPU is used: --gpus N > 1 exits with a message.
+Prints one JSON line.  Beside the headline it carries the FP64 line, the roofline of the
+magnetic kernel, event-timed rows of the other operations, the ensemble-flow workload
+(32 configurations of 16^3 x Nt, 100 RK3 steps in one hipGraph) and, unless --no-cpu,
+the CPU oracle on the same 48^3x96 field: its time is the CPU baseline and its B,
+plaquette and Polyakov sums give each observable's error (rel_err_vs_oracle).
+Only one GPU is used: --gpus N > 1 exits with a message.
 """
 import argparse
 import json
@@ -105,28 +110,53 @@ def gpu_rows(fields, torch):
     return out
 
 
-def cpu_baseline(nslices=8, min_seconds=12.0):
-    """The CPU oracle (FP64, OpenMP) on a bounded sample of the same workload: a 48^3 x
-    nslices lattice from the same generator, measured repeatedly until min_seconds of
-    CPU work have passed.  B uses spatial planes only, so each time slice is the same
-    work per site as a slice of the full 48^3x96 lattice."""
+def oracle_leg(f64, torch):
+    """The CPU oracle (FP64, OpenMP) on the whole benchmark workload: the 48^3x96 field the
+    GPU measured, exported unchanged to the host.  Its B pass is the CPU baseline; its B,
+    plaquette and Polyakov sums are what the GPU results are compared with."""
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    dims = (nslices,) + DIMS[1:]
-    U = O.random_field(dims, SEED, EPS)
-    reps = 0
+    U = f64.to_complex().cpu().numpy()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    while True:
-        O.magnetic_sum_tslices(U, 1, nslices)
-        reps += 1
+    B = O.magnetic(U)
+    dt = time.perf_counter() - t0
+    V = DIMS[0] * DIMS[1] * DIMS[2] * DIMS[3]
+    ref = {"B": B, "plaquette": O.plaquette(U, 1, 4, 4)[0], "polyakov": O.polyakov(U)[0]}
+    small = O.random_field((8,) + DIMS[1:], SEED, EPS)
+    del U
+    base = {"value": V / dt / 1e9, "unit": "Gsites/s", "cores": threads, "kind": "port",
+            "sample": f"oracle magnetic (FP64, {threads} OpenMP threads) over the whole 48^3x96 lattice: "
+                      f"{V} sites in {dt:.2f} s",
+            "rows": cpu_rows(small, threads)}
+    return base, ref
+
+
+def ensemble_leg(FL, torch):
+    """The batching workload: 32 configurations 16^3 x Nt, Nt drawn from {16, 24, 32, 48, 64,
+    128} (seeded), links eps = 0.4 (seeds 4000 + j), 100 RK3 Wilson-flow steps at dt = 0.01
+    with the plaquette and B of every configuration every 10 steps, all in one hipGraph.
+    Timed around the whole call (capture, launch, and the copy of the measurements back)."""
+    import numpy as np
+    from flue_lat_amd.ensemble import Ensemble
+    nts = np.random.default_rng(4000).choice([16, 24, 32, 48, 64, 128], size=32)
+    dims = [(int(nt), 16, 16, 16) for nt in nts]
+    seeds = [4000 + j for j in range(32)]
+    sites = int(sum(np.prod(d) for d in dims))
+    out = {"configs": 32, "nt": [int(v) for v in nts], "sites": sites, "steps": 100, "dt": 0.01,
+           "measure_every": 10}
+    for prec in ("fp32", "fp64"):
+        e = Ensemble.random(dims, seeds, 0.4, prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plaq, B = e.flow(0.01, 100, measure_every=10, use_graph=True)
         dt = time.perf_counter() - t0
-        if dt >= min_seconds:
-            break
-    sites = reps * nslices * DIMS[1] * DIMS[2] * DIMS[3]
-    return {"value": sites / dt / 1e9, "unit": "Gsites/s", "cores": threads, "kind": "port",
-            "sample": f"oracle magnetic (FP64, {threads} OpenMP threads) over {reps} passes of a "
-                      f"48^3x{nslices} lattice: {sites} sites in {dt:.2f} s",
-            "rows": cpu_rows(U, threads)}
+        out[prec] = {"s": round(dt, 4), "gsite_steps_per_s": round(sites * 100 / dt / 1e9, 4),
+                     "mean_plaquette_final": float(np.mean(plaq[-1] / (18.0 * np.array([np.prod(d) for d in dims])))),
+                     "mean_B_final": float(np.mean(B[-1]))}
+        del e
+        torch.cuda.empty_cache()
+    return out
 
 
 def hbm_traffic():
@@ -191,7 +221,7 @@ def main():
     for name, g in (("fp32", f32), ("fp64", f64)):
         s, n = g.plaquette(1, 4, 4)
         p, npl = g.polyakov()
-        extras[name] = {"plaquette": s / (3 * n), "polyakov": p / (3 * npl)}
+        extras[name] = {"plaquette": s / (3 * n), "polyakov": p / (3 * npl), "plaquette_sum": s, "polyakov_sum": p}
     rel_B = abs(res["fp32"]["B"] - res["fp64"]["B"]) / abs(res["fp64"]["B"])
 
     traffic = hbm_traffic()
@@ -243,8 +273,17 @@ def main():
                      "frac": round(FLOP_PER_SITE * V / res["fp64"]["s_per_step"] / 1e12 / PEAK_TFLOPS["fp64"], 4)},
         },
     }
+    line["ensemble"] = ensemble_leg(FL, torch)
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
+        base, ref = oracle_leg(f64, torch)
+        line["cpu_baseline"] = base
+        got = {"fp32": {"B": res["fp32"]["B"]}, "fp64": {"B": res["fp64"]["B"]}}
+        for name in ("fp32", "fp64"):
+            got[name]["plaquette"] = extras[name]["plaquette_sum"]
+            got[name]["polyakov"] = extras[name]["polyakov_sum"]
+        line["rel_err_vs_oracle"] = {name: {k: abs(got[name][k] - ref[k]) / abs(ref[k]) for k in ref}
+                                     for name in got}
+        line["oracle"] = ref
     print(json.dumps(line))
 
 

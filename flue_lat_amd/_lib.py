@@ -16,8 +16,9 @@ FLUE_OUT_STRIDED, FLUE_OUT_SOA = 0, 1
 
 STATUS = {
     0: "FLUE_OK", 1: "FLUE_EINVAL", 2: "FLUE_ENULL", 3: "FLUE_EPATH", 4: "FLUE_ENT", 5: "FLUE_EDIM",
-    6: "FLUE_EHIP", 7: "FLUE_ENOMEM", 8: "FLUE_ENOTIMPL", 9: "FLUE_ENODEV", 10: "FLUE_EWS",
+    6: "FLUE_EHIP", 7: "FLUE_ENOMEM", 8: "FLUE_ENOTIMPL", 9: "FLUE_ENODEV", 10: "FLUE_EWS", 11: "FLUE_ESU3",
 }
+FLUE_ESU3 = 11
 
 
 class FlueError(RuntimeError):
@@ -66,6 +67,7 @@ SIGNATURES = {
     "flue_export_field": [_P, _P, _I, _P, _I, _P, _P],
     "flue_convert_field": [_P, _P, _I, _P, _I, _P],
     "flue_random_field": [_P, ctypes.c_uint64, _D, _P, _I, _P],
+    "flue_check_field": [_P, _P, _I, _D, _P, _P, _SZ, _P],
     "flue_plaquette": [_P, _P, _I, _I, _I, _I, _P, _P, _SZ, _P],
     "flue_polyakov": [_P, _P, _I, _P, _P, _SZ, _P],
     "flue_path_trace": [_P, _P, _I, _P, _I, _I, _P, _P, _SZ, _P],

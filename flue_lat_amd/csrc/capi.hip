@@ -86,6 +86,28 @@ size_t partial_bytes(uint32_t nblocks) {
     return ((size_t)nblocks + (nblocks + kFinalChunk - 1) / kFinalChunk) * 2 * sizeof(double) + 256;
 }
 
+// ---------------------------------------------------------------- SU(3) check
+// per-block (count, max, max) triples and the combined triple
+size_t check_bytes(const Lat& L) { return ((size_t)check_blocks(L) * 3 + 3) * sizeof(double) + 256; }
+
+// largest accepted max|U U^+ - 1| and |det U - 1|: a few hundred FP32 roundings of a unit
+// entry for FP32 links, far above FP64 rounding for FP64 links
+double su3_bound(int32_t prec) { return prec == FLUE_F32 ? 1e-5 : 1e-10; }
+
+// scans U with `w` (check_bytes) as scratch, waits for the result, and returns FLUE_ESU3
+// when a link has a non-finite entry or exceeds the bound (tol == 0: su3_bound(prec))
+int run_su3_check(const Lat& L, const void* U, int32_t prec, double tol, double* stats, void* w, hipStream_t st) {
+    double* part = (double*)w;
+    double* out = part + 3 * (size_t)check_blocks(L);
+    FLUE_TRY(launch_check_field(L, U, prec == FLUE_F32, part, out, st));
+    double h[3];
+    FLUE_TRY(hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, st));
+    FLUE_TRY(hipStreamSynchronize(st));
+    if (stats) std::memcpy(stats, h, sizeof h);
+    const double b = tol > 0.0 ? tol : su3_bound(prec);
+    return (h[0] > 0.0 || !(h[1] <= b) || !(h[2] <= b)) ? FLUE_ESU3 : FLUE_OK;
+}
+
 // ---------------------------------------------------------------- autotuner table
 // (device, op, precision, NT, NX, NY, NZ) -> fastest kernel variant measured on it
 struct TuneKey {
@@ -209,6 +231,7 @@ const char* flue_strerror(int s) {
     case FLUE_ENOTIMPL: return "outside the scope of this library";
     case FLUE_ENODEV: return "no GPU visible";
     case FLUE_EWS: return "workspace too small";
+    case FLUE_ESU3: return "links are not SU(3): a non-finite entry, or |U U^+ - 1| or |det U - 1| above the bound";
     default: return "unknown status";
     }
 }
@@ -358,7 +381,20 @@ size_t flue_workspace_bytes(const int32_t dims[4]) {
     nb = std::max(nb, path_blocks(L, -1, nullptr, 0));
     nb = std::max(nb, magnetic_blocks(L, 0));
     nb = std::max(nb, magnetic_blocks(L, 1));
-    return std::max({partial_bytes(nb), wloop_partial_bytes(L, -1) + 256, wloop_partial_bytes(L, kEverySlice) + 256});
+    return std::max({partial_bytes(nb), wloop_partial_bytes(L, -1) + 256, wloop_partial_bytes(L, kEverySlice) + 256,
+                     check_bytes(L)});
+}
+
+int flue_check_field(const int32_t dims[4], const void* U, int32_t prec, double tol, double stats[3], void* ws,
+                     size_t ws_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U) return FLUE_ENULL;
+    if (check_prec(prec) || std::isnan(tol) || tol < 0.0) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, check_bytes(L), &w))) return rc;
+    return run_su3_check(L, U, prec, tol, stats, w, S(stream));
 }
 
 int flue_import_field(const int32_t dims[4], const void* src, int32_t src_type, const int64_t strides[7], void* dst,
@@ -534,12 +570,12 @@ int flue_gluon_prop_mom(const int32_t dims[4], const void* Uq, int32_t prec, con
 int flue_scalar_gluon_prop(const int32_t dims[4], const void* U, int32_t prec, double* d_D, void* stream) {
     int rc = check_dims(dims);
     if (rc) return rc;
-    if (!U || !d_D) return FLUE_ENULL;
-    if (check_prec(prec)) return FLUE_EINVAL;
-    // D(NS,NS,NS,NS) is written at every (t,x,y,z) of the field
+    // sizes first: D(NS,NS,NS,NS) is written at every (t,x,y,z) of the field
     if (dims[2] != dims[1] || dims[3] != dims[1] || dims[0] > dims[1]) return FLUE_EDIM;
     const uint64_t ns = (uint64_t)dims[1];
     if (ns * ns * ns * ns > kMaxSites) return FLUE_EDIM;
+    if (!U || !d_D) return FLUE_ENULL;
+    if (check_prec(prec)) return FLUE_EINVAL;
     return hip_status(launch_scalar_gluon(make_lat(dims), U, prec == FLUE_F32, dims[1], d_D, S(stream)));
 }
 
@@ -571,7 +607,8 @@ int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t
 size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec) {
     if (check_dims(dims) || check_prec(prec)) return 0;
     const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
-    return (size_t)(27 * V * (prec == FLUE_F32 ? 4 : 8));
+    // the per-site anti-Hermitian clover sums; the SU(3) check uses the same buffer first
+    return std::max((size_t)(27 * V * (prec == FLUE_F32 ? 4 : 8)), check_bytes(make_lat(dims)));
 }
 
 size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec) {
@@ -614,6 +651,9 @@ int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad,
     const Lat L = make_lat(dims);
     void* A;
     if ((rc = get_ws(work, work_bytes, flue_b2_work_bytes(dims, prec), &A))) return rc;
+    // the gather takes the product from a corner to the walk's end as X_c^+ F, which holds
+    // for unitary links only: refuse fields that are not SU(3) to the precision's bound
+    if ((rc = run_su3_check(L, U, prec, 0.0, nullptr, A, S(stream)))) return rc;
     FLUE_TRY(launch_magnetic_ah(L, U, prec == FLUE_F32, A, S(stream)));
     FLUE_TRY(launch_b2_grad(L, U, prec == FLUE_F32, A, grad, S(stream)));
     return FLUE_OK;

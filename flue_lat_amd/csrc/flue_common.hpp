@@ -70,8 +70,11 @@ struct FieldView {
     __device__ __forceinline__ const R* cplane(int mu, int e) const { return p + (size_t)(mu * 9 + e) * 2 * V; }
     __device__ __forceinline__ __amdgpu_buffer_rsrc_t dir_rsrc(int mu) const {
         const char* b = (const char*)p + (size_t)mu * 18u * V * sizeof(R);
+        // num_records is the unsigned 32-bit byte count of the 9 planes (fits_buffer keeps it
+        // below 2^32); the builtin takes it as an int, so its bit pattern is passed unchanged.
         // dword3 0x00020000: 32-bit data format, no swizzle -- the raw-buffer setting for gfx9 parts
-        return __builtin_amdgcn_make_buffer_rsrc((void*)b, (short)0, (int)(18u * V * (uint32_t)sizeof(R)), 0x00020000);
+        const uint32_t nrec = 18u * V * (uint32_t)sizeof(R);
+        return __builtin_amdgcn_make_buffer_rsrc((void*)b, (short)0, __builtin_bit_cast(int, nrec), 0x00020000);
     }
     // one (re, im) pair: a 64-bit (FP32) or 128-bit (FP64) load, reinterpreted as a whole
     // (taking dwords out of the loaded vector one by one is not used: with bit casts of

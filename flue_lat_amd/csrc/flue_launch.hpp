@@ -17,6 +17,10 @@ hipError_t launch_export(const Lat& L, const void* src, int src_is_f32, void* ds
                          const int64_t strides[7], hipStream_t st);
 hipError_t launch_random_field(const Lat& L, uint64_t seed, double eps, void* dst, int is_f32, hipStream_t st);
 hipError_t launch_convert(const Lat& L, const void* src, int src_is_f32, void* dst, int dst_is_f32, hipStream_t st);
+// SU(3) check of every link: part holds 3 doubles per block (check_blocks), out3 receives
+// (links with a non-finite entry, max |U U^+ - 1|, max |det U - 1|)
+uint32_t check_blocks(const Lat& L);
+hipError_t launch_check_field(const Lat& L, const void* U, int is_f32, double* part, double* out3, hipStream_t st);
 
 // reductions: each writes one compensated partial (hi, lo) per block to `partials`,
 // then launch_finalize sums them in a fixed order into out[0] (double, device)

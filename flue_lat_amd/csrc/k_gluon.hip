@@ -190,7 +190,9 @@ __global__ void __launch_bounds__(kBS) scalar_gluon_kernel(Lat L, FV f, int NS, 
 
 hipError_t launch_mom_scalar(const Lat& Lq, const void* Uq, int is_f32, int mu_start, double xi,
                              const int32_t dext[4], const int64_t dstr[4], double* D, hipStream_t st) {
-    const uint32_t nD = (uint32_t)dext[0] * dext[1] * dext[2] * dext[3];
+    const uint64_t nD64 = (uint64_t)dext[0] * dext[1] * dext[2] * dext[3];
+    if (nD64 > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const uint32_t nD = (uint32_t)nD64;
     const uint32_t g = (nD + kBS - 1) / kBS;
     const int4 de = make_int4(dext[0], dext[1], dext[2], dext[3]);
     const longlong4 ds = make_longlong4(dstr[0], dstr[1], dstr[2], dstr[3]);
@@ -201,7 +203,10 @@ hipError_t launch_mom_scalar(const Lat& Lq, const void* Uq, int is_f32, int mu_s
 }
 
 hipError_t launch_scalar_gluon(const Lat& L, const void* U, int is_f32, int NS, double* D, hipStream_t st) {
-    const uint32_t nD = (uint32_t)NS * NS * NS * NS;
+    // D holds NS^4 entries indexed in 32 bits: refuse anything the grid cannot cover
+    const uint64_t nD64 = (uint64_t)NS * NS * NS * NS;
+    if (NS < 1 || nD64 > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const uint32_t nD = (uint32_t)nD64;
     const uint32_t g = (nD + kBS - 1) / kBS;
     with_view(L.V, U, is_f32, [&](auto f) {
         scalar_gluon_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, NS, nD, D);

@@ -14,7 +14,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import FLUE_C64, FLUE_C128, FLUE_F32, FLUE_F64, FLUE_F64_SU3, FLUE_OUT_SOA, FLUE_OUT_STRIDED, check
+from ._lib import FLUE_C64, FLUE_C128, FLUE_F32, FLUE_F64, FLUE_F64_SU3, FLUE_OUT_SOA, FLUE_OUT_STRIDED
+from ._lib import check as check_status
 
 # "fp64-su3": FP64 links the caller declares SU(3); the magnetic measurement may then carry
 # two rows of each product, as FP32 always does
@@ -54,12 +55,12 @@ def require_gpu():
 
 def tune_save(path):
     """Write the autotuner's table (fastest kernel variant per device, op, precision, extents)."""
-    check(_lib.lib().flue_tune_save(str(path).encode()), "flue_tune_save")
+    check_status(_lib.lib().flue_tune_save(str(path).encode()), "flue_tune_save")
 
 
 def tune_load(path):
     """Merge a table written by tune_save into the autotuner."""
-    check(_lib.lib().flue_tune_load(str(path).encode()), "flue_tune_load")
+    check_status(_lib.lib().flue_tune_load(str(path).encode()), "flue_tune_load")
 
 
 class GaugeField:
@@ -68,8 +69,12 @@ class GaugeField:
     dims = (NT, NX, NY, NZ) as in the reference's data(NT,NX,NY,NZ,4,3,3).
     """
 
-    def __init__(self, dims, precision="fp64", device=None, _data=None):
+    def __init__(self, dims, precision="fp64", device=None, _data=None, check=True):
+        """A field of the given extents; with _data, a view of existing links in the SoA
+        layout.  Declaring existing links "fp64-su3" runs check_su3() on them first (raises
+        FlueError FLUE_ESU3 on links that are not SU(3)); check=False skips that."""
         require_gpu()
+        declared = _data is not None   # existing links declared in this precision
         self.dims = tuple(int(d) for d in dims)
         if len(self.dims) != 4 or min(self.dims) < 1:
             raise ValueError(f"dims must be 4 positive extents (NT, NX, NY, NZ), got {dims}")
@@ -82,6 +87,8 @@ class GaugeField:
         self.data = _data
         self._cdims = _i32(*self.dims)
         self._ws = None
+        if check and self.prec == FLUE_F64_SU3 and declared:
+            self.check_su3()
 
     # ------------------------------------------------------------ construction
     @property
@@ -99,14 +106,15 @@ class GaugeField:
         """Seeded synthetic field U = fix(exp(i eps sum_a c_a lambda_a/2)), generated on the GPU."""
         f = cls(dims, precision, device)
         with torch.cuda.device(f.device):
-            check(_lib.lib().flue_random_field(f._cdims, ctypes.c_uint64(seed), float(eps), _ptr(f.data), f.prec,
+            check_status(_lib.lib().flue_random_field(f._cdims, ctypes.c_uint64(seed), float(eps), _ptr(f.data), f.prec,
                                                _stream()), "flue_random_field")
         return f
 
     @classmethod
-    def from_complex(cls, U, precision="fp64", device=None):
+    def from_complex(cls, U, precision="fp64", device=None, check=True):
         """From a (NT,NX,NY,NZ,4,3,3) complex array: numpy (any order) or a torch tensor
-        (any strides, complex128 or complex64, CPU or GPU)."""
+        (any strides, complex128 or complex64, CPU or GPU).  For "fp32" and "fp64-su3" the
+        imported links are checked with check_su3() unless check=False."""
         if isinstance(U, np.ndarray):
             U = torch.from_numpy(np.asarray(U))
         if U.dim() != 7 or tuple(U.shape[4:]) != (4, 3, 3):
@@ -117,10 +125,12 @@ class GaugeField:
         src = U.to(f.device) if U.device != f.device else U
         with torch.cuda.device(f.device):
             src_type = FLUE_C64 if src.dtype == torch.complex64 else FLUE_C128
-            check(_lib.lib().flue_import_field(f._cdims, _ptr(src), src_type, _i64(src.stride()), _ptr(f.data), f.prec,
+            check_status(_lib.lib().flue_import_field(f._cdims, _ptr(src), src_type, _i64(src.stride()), _ptr(f.data), f.prec,
                                                _stream()), "flue_import_field")
             if src is not U:
                 torch.cuda.current_stream().synchronize()
+        if check and f.prec in (FLUE_F32, FLUE_F64_SU3):
+            f.check_su3()
         return f
 
     def to_complex(self, dtype=torch.complex128):
@@ -130,17 +140,37 @@ class GaugeField:
         out = buf.permute(6, 5, 4, 3, 2, 1, 0)
         with torch.cuda.device(self.device):
             dst_type = FLUE_C64 if dtype == torch.complex64 else FLUE_C128
-            check(_lib.lib().flue_export_field(self._cdims, _ptr(self.data), self.prec, _ptr(out), dst_type,
+            check_status(_lib.lib().flue_export_field(self._cdims, _ptr(self.data), self.prec, _ptr(out), dst_type,
                                                _i64(out.stride()), _stream()), "flue_export_field")
         return out
 
-    def astype(self, precision):
+    def astype(self, precision, check=True):
+        """The field in another precision.  FP32 links and "fp64-su3" links must be SU(3)
+        (their kernels rebuild row 2 of link products from rows 0 and 1): the result is
+        checked with check_su3() and FlueError (FLUE_ESU3) is raised on failure, unless
+        check=False."""
         p = _prec(precision)
         g = GaugeField(self.dims, _PREC_NAME[p], self.device)
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_convert_field(self._cdims, _ptr(self.data), self.prec, _ptr(g.data), g.prec,
-                                                _stream()), "flue_convert_field")
+            check_status(_lib.lib().flue_convert_field(self._cdims, _ptr(self.data), self.prec, _ptr(g.data), g.prec,
+                                                       _stream()), "flue_convert_field")
+        if check and p in (FLUE_F32, FLUE_F64_SU3):
+            g.check_su3()
         return g
+
+    def check_su3(self, tol=0.0, raise_on_fail=True):
+        """Deterministic scan of every link in FP64: returns {"nonfinite": links with a NaN or
+        infinite entry, "unitarity": max |(U U^+ - 1)_ab|, "det": max |det U - 1|}.  Raises
+        FlueError (FLUE_ESU3) when a link is non-finite or a deviation exceeds tol (0 = the
+        precision's bound: 1e-5 for FP32 links, 1e-10 for FP64) unless raise_on_fail=False."""
+        stats = (ctypes.c_double * 3)()
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            rc = _lib.lib().flue_check_field(self._cdims, _ptr(self.data), self.prec, float(tol), stats, _ptr(ws),
+                                             ws.numel(), _stream())
+        if rc != _lib.FLUE_ESU3 or raise_on_fail:
+            check_status(rc, "flue_check_field")
+        return {"nonfinite": int(stats[0]), "unitarity": stats[1], "det": stats[2]}
 
     # ------------------------------------------------------------ observables
     def _scalar(self):
@@ -151,7 +181,7 @@ class GaugeField:
         out = self._scalar()
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_plaquette(self._cdims, _ptr(self.data), self.prec, muStart, muEnd, nuEnd, _ptr(out),
+            check_status(_lib.lib().flue_plaquette(self._cdims, _ptr(self.data), self.prec, muStart, muEnd, nuEnd, _ptr(out),
                                             _ptr(ws), ws.numel(), _stream()), "flue_plaquette")
         nP = sum(max(0, nuEnd - mu) for mu in range(muStart, muEnd + 1)) * self.V
         return (out.item() if sync else out), nP
@@ -161,7 +191,7 @@ class GaugeField:
         out = self._scalar()
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_polyakov(self._cdims, _ptr(self.data), self.prec, _ptr(out), _ptr(ws), ws.numel(),
+            check_status(_lib.lib().flue_polyakov(self._cdims, _ptr(self.data), self.prec, _ptr(out), _ptr(ws), ws.numel(),
                                            _stream()), "flue_polyakov")
         return (out.item() if sync else out), self.V // self.dims[0]
 
@@ -171,7 +201,7 @@ class GaugeField:
         out = self._scalar()
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_path_trace(self._cdims, _ptr(self.data), self.prec, p, len(path), int(tau), _ptr(out),
+            check_status(_lib.lib().flue_path_trace(self._cdims, _ptr(self.data), self.prec, p, len(path), int(tau), _ptr(out),
                                              _ptr(ws), ws.numel(), _stream()), "flue_path_trace")
         return out.item() if sync else out
 
@@ -187,7 +217,7 @@ class GaugeField:
         out = torch.empty(int(Rmax) * int(Tmax), dtype=torch.float64, device=self.device)
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_wilson_loops(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu), int(Rmax),
+            check_status(_lib.lib().flue_wilson_loops(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu), int(Rmax),
                                                int(Tmax), int(tau), _ptr(out), _ptr(ws), ws.numel(), _stream()),
                   "flue_wilson_loops")
         return out.cpu().numpy().reshape(int(Rmax), int(Tmax))
@@ -199,7 +229,7 @@ class GaugeField:
         out = torch.empty(int(Rmax) * int(Tmax) * NT, dtype=torch.float64, device=self.device)
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_wilson_loops_slices(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu),
+            check_status(_lib.lib().flue_wilson_loops_slices(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu),
                                                       int(Rmax), int(Tmax), _ptr(out), _ptr(ws), ws.numel(),
                                                       _stream()), "flue_wilson_loops_slices")
         return out.cpu().numpy().reshape(int(Rmax), int(Tmax), NT)
@@ -209,7 +239,7 @@ class GaugeField:
         out = torch.empty(18, dtype=torch.float64, device=self.device)
         p = _i32(*[int(v) for v in path])
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_generic_path(self._cdims, _ptr(self.data), self.prec, _i32(*coordBase), p, len(path),
+            check_status(_lib.lib().flue_generic_path(self._cdims, _ptr(self.data), self.prec, _i32(*coordBase), p, len(path),
                                                _ptr(out), _stream()), "flue_generic_path")
         v = out.cpu().numpy()
         return (v[0::2] + 1j * v[1::2]).reshape(3, 3, order="F")
@@ -221,7 +251,7 @@ class GaugeField:
         buf = torch.empty((3, 3, NZ, NY, NX, NT), dtype=torch.complex128, device=self.device)
         out = buf.permute(5, 4, 3, 2, 1, 0)
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_fmunu(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu), 1 if clover else 0,
+            check_status(_lib.lib().flue_fmunu(self._cdims, _ptr(self.data), self.prec, int(mu), int(nu), 1 if clover else 0,
                                         _ptr(out), FLUE_OUT_STRIDED, _i64(out.stride()), _stream()), "flue_fmunu")
         return out
 
@@ -247,7 +277,7 @@ class GaugeField:
         dext = tuple(d // 2 + 1 for d in self.dims)
         D = torch.empty(dext, dtype=torch.complex128, device=self.device)
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_gluon_prop_mom(self._cdims, _ptr(Uq), self.prec, qstrides, int(mu_start),
+            check_status(_lib.lib().flue_gluon_prop_mom(self._cdims, _ptr(Uq), self.prec, qstrides, int(mu_start),
                                                  float(xi), _i32(*dext), _ptr(D), _i64(D.stride()), _stream()),
                   "flue_gluon_prop_mom")
         return D
@@ -258,7 +288,7 @@ class GaugeField:
         NS = self.dims[1]
         buf = torch.empty(NS ** 4, dtype=torch.float64, device=self.device)
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_scalar_gluon_prop(self._cdims, _ptr(self.data), self.prec, _ptr(buf), _stream()),
+            check_status(_lib.lib().flue_scalar_gluon_prop(self._cdims, _ptr(self.data), self.prec, _ptr(buf), _stream()),
                   "flue_scalar_gluon_prop")
         # Fortran order: t fastest
         return buf.view(NS, NS, NS, NS).permute(3, 2, 1, 0)
@@ -270,7 +300,7 @@ class GaugeField:
         out = self._scalar()
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_magnetic_ex(self._cdims, _ptr(self.data), self.prec, int(variant), _ptr(out),
+            check_status(_lib.lib().flue_magnetic_ex(self._cdims, _ptr(self.data), self.prec, int(variant), _ptr(out),
                                               _ptr(ws), ws.numel(), _stream()), "flue_magnetic_ex")
         return out.item() if sync else out
 
@@ -282,7 +312,7 @@ class GaugeField:
         ms = ctypes.c_double()
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_autotune(ops[op], self._cdims, _ptr(self.data), self.prec, int(reps),
+            check_status(_lib.lib().flue_autotune(ops[op], self._cdims, _ptr(self.data), self.prec, int(reps),
                                            ctypes.byref(best), ctypes.byref(ms), _ptr(ws), ws.numel(), _stream()),
                   "flue_autotune")
         return best.value, ms.value
@@ -296,7 +326,7 @@ class GaugeField:
         out = self._scalar()
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_wilson_action(self._cdims, _ptr(self.data), self.prec, float(beta), _ptr(out),
+            check_status(_lib.lib().flue_wilson_action(self._cdims, _ptr(self.data), self.prec, float(beta), _ptr(out),
                                                 _ptr(ws), ws.numel(), _stream()), "flue_wilson_action")
         return out.item() if sync else out
 
@@ -305,17 +335,19 @@ class GaugeField:
         su(3) field (same SoA layout and precision)."""
         g = self._like()
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_wilson_grad(self._cdims, _ptr(self.data), self.prec, float(beta), _ptr(g.data),
+            check_status(_lib.lib().flue_wilson_grad(self._cdims, _ptr(self.data), self.prec, float(beta), _ptr(g.data),
                                               _stream()), "flue_wilson_grad")
         return g
 
     def magnetic_grad(self):
-        """Lie gradient of the magnetic B (temporal links get 0)."""
+        """Lie gradient of the magnetic B (temporal links get 0).  Needs SU(3) links in every
+        precision: the library checks them first and FlueError (FLUE_ESU3) is raised for a
+        field that fails the check."""
         g = self._like()
         n = _lib.lib().flue_b2_work_bytes(self._cdims, self.prec)
         work = torch.empty(n, dtype=torch.uint8, device=self.device)
         with torch.cuda.device(self.device):
-            check(_lib.lib().flue_b2_grad(self._cdims, _ptr(self.data), self.prec, _ptr(g.data), _ptr(work), n,
+            check_status(_lib.lib().flue_b2_grad(self._cdims, _ptr(self.data), self.prec, _ptr(g.data), _ptr(work), n,
                                           _stream()), "flue_b2_grad")
         return g
 
@@ -330,11 +362,11 @@ class GaugeField:
                 side = torch.cuda.Stream(device=self.device)
                 side.wait_stream(stream)
                 with torch.cuda.stream(side):
-                    check(_lib.lib().flue_wilson_flow(self._cdims, _ptr(self.data), self.prec, float(eps), int(nsteps),
+                    check_status(_lib.lib().flue_wilson_flow(self._cdims, _ptr(self.data), self.prec, float(eps), int(nsteps),
                                                       1, _ptr(work), n, _stream()), "flue_wilson_flow")
                 stream.wait_stream(side)
             else:
-                check(_lib.lib().flue_wilson_flow(self._cdims, _ptr(self.data), self.prec, float(eps), int(nsteps),
+                check_status(_lib.lib().flue_wilson_flow(self._cdims, _ptr(self.data), self.prec, float(eps), int(nsteps),
                                                   1 if use_graph else 0, _ptr(work), n, _stream()), "flue_wilson_flow")
         return self
 

@@ -19,8 +19,9 @@
  *    SoA layout: 36 complex planes (direction mu, colour a, b) of V sites, each
  *    site an adjacent (re, im) pair: real index (((mu*3 + a)*3 + b)*V + s)*2 + ri
  *    with s = x + NX*(y + NY*(z + NZ*t)), FP64 or FP32 (72 V reals in all).
- *    They are asynchronous on the given hipStream_t (NULL = default stream),
- *    write results to device memory and never synchronise.  Lattice
+ *    They are asynchronous on the given hipStream_t (NULL = default stream) and
+ *    write results to device memory; the few that wait for the stream say so
+ *    (flue_check_field, flue_b2_grad, and the graph-launching flow calls).  Lattice
  *    sums are deterministic (fixed reduction order) and use compensated FP64
  *    partial sums; `ws` is a device workspace of at least
  *    flue_workspace_bytes(dims) bytes (NULL = a buffer the library keeps per
@@ -56,11 +57,14 @@ enum {
     FLUE_ENOMEM = 7,   /* device allocation failed */
     FLUE_ENOTIMPL = 8, /* outside this library's scope (kept for the ABI; no entry returns it) */
     FLUE_ENODEV = 9,   /* no GPU visible */
-    FLUE_EWS = 10      /* workspace too small */
+    FLUE_EWS = 10,     /* workspace too small */
+    FLUE_ESU3 = 11     /* links are not SU(3) within the bound (see flue_check_field) */
 };
 
-/* link precision of a SoA field.  FLUE_F64 makes no assumption on the links.  FLUE_F32
- * fields must hold SU(3) links (unitary, det 1, to FP32 rounding): the clover kernels
+/* link precision of a SoA field.  FLUE_F64 makes no assumption on the links, as the
+ * reference does not, except flue_b2_grad (below), which needs SU(3) links in every
+ * precision and checks for them.  FLUE_F32 fields must hold SU(3) links (unitary, det 1,
+ * to FP32 rounding; flue_check_field tests it): the clover kernels
  * carry two rows of each link product and rebuild the third as conj(row0 x row1).
  * Sums and per-site traces are FP64 in both.  FLUE_F64_SU3 is an FP64 field whose links
  * the caller declares SU(3) (to FP64 rounding): stored and computed as FLUE_F64, except
@@ -188,6 +192,16 @@ int flue_export_field(const int32_t dims[4], const void* src, int32_t prec, void
 int flue_convert_field(const int32_t dims[4], const void* src, int32_t src_prec, void* dst, int32_t dst_prec,
                        void* stream);
 
+/* SU(3) check of every link, computed in FP64 (no reference counterpart; the reference
+ * readers reunitarise every link instead, src/gaugeIO/ILDG_bin.f90:138-151): stats[0] = the
+ * number of links with a non-finite entry, stats[1] = max over links and entries of
+ * |(U U^+ - 1)_ab|, stats[2] = max |det U - 1| (stats may be NULL).  Deterministic.  Waits
+ * for `stream` and returns FLUE_ESU3 if stats[0] > 0 or stats[1] or stats[2] exceeds tol;
+ * tol = 0 selects the bound of the precision: 1e-5 for FLUE_F32, 1e-10 for FLUE_F64 and
+ * FLUE_F64_SU3.  ws needs flue_workspace_bytes(dims) bytes. */
+int flue_check_field(const int32_t dims[4], const void* U, int32_t prec, double tol, double stats[3], void* ws,
+                     size_t ws_bytes, void* stream);
+
 /* seeded synthetic field U = fix(exp(i eps sum_a c_a lambda_a/2)), c_a ~ N(0,1) keyed
  * by (seed, link) through Philox4x32-10; no reference counterpart (the reference
  * reads fields from disk, src/gaugeIO/) */
@@ -285,7 +299,11 @@ int flue_wilson_grad(const int32_t dims[4], const void* U, int32_t prec, double 
 
 /* gradient of the magnetic B (flue_magnetic); temporal links get 0.  `work` holds the
  * per-site anti-Hermitian parts of the three spatial-plane leaf sums
- * (flue_b2_work_bytes; NULL = library buffer) */
+ * (flue_b2_work_bytes; NULL = library buffer).  Precondition in every precision: SU(3)
+ * links, since the gather takes the product from a leaf corner to the walk's end as
+ * X^+ F.  The call first runs flue_check_field's scan with the precision's bound (waiting
+ * for `stream` once) and returns FLUE_ESU3, with no gradient written, on a field that
+ * fails it. */
 int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad, void* work, size_t work_bytes,
                  void* stream);
 

@@ -199,33 +199,50 @@ int or_plaquette(const cplx* data, const int32_t dims[4], int32_t muStart, int32
                  double* sumTrP, int32_t* nP) {
     if (check_dims(dims)) return ORC_EINVAL;
     if (muStart < 1 || muEnd > 4 || nuEnd > 4) return ORC_EINVAL;
-    double s = 0.0;
-    int32_t n = 0;
+    /* The reference runs this loop nest serially.  Here the (plane, x) rows run in
+     * parallel, each summed serially in the reference's (y, z, t) order into its own
+     * slot, and the slots are added in the reference's (plane, x) order afterwards, so
+     * the result does not depend on the thread count. */
+    int npl = 0, pmu[6], pnu[6];
     for (int mu = muStart; mu <= muEnd; ++mu)
-        for (int nu = mu + 1; nu <= nuEnd; ++nu)
-            for (int x = 1; x <= dims[1]; ++x)
-                for (int y = 1; y <= dims[2]; ++y)
-                    for (int z = 1; z <= dims[3]; ++z)
-                        for (int t = 1; t <= dims[0]; ++t) {
-                            int32_t base[4] = {t, x, y, z}, c[4];
-                            cplx Umu_x[9], Unu_xmu[9], Umu_xnu[9], Unu_x[9], A[9], B[9];
-                            load_link(data, dims, base, mu, Umu_x); /* :963 */
-                            memcpy(c, base, sizeof c);
-                            c[mu - 1] += 1;
-                            or_period_coord(c, dims, c); /* :968-974 */
-                            load_link(data, dims, c, nu, Unu_xmu);
-                            memcpy(c, base, sizeof c);
-                            c[nu - 1] += 1;
-                            or_period_coord(c, dims, c); /* :980-985 */
-                            load_link(data, dims, c, mu, Umu_xnu);
-                            load_link(data, dims, base, nu, Unu_x); /* :992 */
-                            or_matmul(A, Umu_x, Unu_xmu);          /* :997 */
-                            or_matdagmatdag(B, Umu_xnu, Unu_x);    /* :999 */
-                            s += or_retrace_mult(A, B);            /* :1001,1005 */
-                            n += 1;
-                        }
+        for (int nu = mu + 1; nu <= nuEnd; ++nu) {
+            pmu[npl] = mu;
+            pnu[npl] = nu;
+            ++npl;
+        }
+    const int nrow = npl * dims[1];
+    double* row = (double*)calloc(nrow > 0 ? (size_t)nrow : 1, sizeof(double));
+    if (!row) return ORC_EINVAL;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int r = 0; r < nrow; ++r) {
+        const int mu = pmu[r / dims[1]], nu = pnu[r / dims[1]], x = r % dims[1] + 1;
+        double s = 0.0;
+        for (int y = 1; y <= dims[2]; ++y)
+            for (int z = 1; z <= dims[3]; ++z)
+                for (int t = 1; t <= dims[0]; ++t) {
+                    int32_t base[4] = {t, x, y, z}, c[4];
+                    cplx Umu_x[9], Unu_xmu[9], Umu_xnu[9], Unu_x[9], A[9], B[9];
+                    load_link(data, dims, base, mu, Umu_x); /* :963 */
+                    memcpy(c, base, sizeof c);
+                    c[mu - 1] += 1;
+                    or_period_coord(c, dims, c); /* :968-974 */
+                    load_link(data, dims, c, nu, Unu_xmu);
+                    memcpy(c, base, sizeof c);
+                    c[nu - 1] += 1;
+                    or_period_coord(c, dims, c); /* :980-985 */
+                    load_link(data, dims, c, mu, Umu_xnu);
+                    load_link(data, dims, base, nu, Unu_x); /* :992 */
+                    or_matmul(A, Umu_x, Unu_xmu);          /* :997 */
+                    or_matdagmatdag(B, Umu_xnu, Unu_x);    /* :999 */
+                    s += or_retrace_mult(A, B);            /* :1001,1005 */
+                }
+        row[r] = s;
+    }
+    double s = 0.0;
+    for (int r = 0; r < nrow; ++r) s += row[r];
+    free(row);
     *sumTrP = s;
-    *nP = n;
+    *nP = (int32_t)((int64_t)npl * vol4(dims));
     return ORC_OK;
 }
 

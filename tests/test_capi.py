@@ -75,10 +75,13 @@ def test_propagator_argument_errors_before_device_work():
     with pytest.raises(FlueError) as e:
         FL.gluonAna.q_average(3, [1.0, 2.0, 3.0], [0.0, 0.0, 0.0], 0)
     assert e.value.status == 1
-    # D(NS^4) past 2^31 entries is refused before the field is touched (the pointer is never read)
+    # D(NS^4) past 2^31 entries is refused by the size check, which comes before the pointer
+    # checks: NULL pointers get FLUE_EDIM here, never a launch
     from flue_lat_amd import _lib
     dims = (ctypes.c_int32 * 4)(1, 256, 256, 256)
-    assert _lib.lib().flue_scalar_gluon_prop(dims, ctypes.c_void_p(64), 0, ctypes.c_void_p(64), None) == 5
+    assert _lib.lib().flue_scalar_gluon_prop(dims, None, 0, None, None) == 5
+    dims = (ctypes.c_int32 * 4)(1, 4, 4, 4)
+    assert _lib.lib().flue_scalar_gluon_prop(dims, None, 0, None, None) == 2
 
 
 def test_argument_errors_before_device_work(L):

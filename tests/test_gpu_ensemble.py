@@ -67,13 +67,18 @@ def test_ensemble_flow_against_oracle(FL):
         assert abs(plaq[1, c] - O.plaquette(W, 1, 4, 4)[0]) < 1e-12 * abs(plaq[1, c])
 
 
-def test_ensemble_c4_shape_fp32(FL):
-    # ensemble workload: 32 configurations of 16^3 x Nt, Nt from {16,24,32,48,64,128}, eps = 0.4, FP32 links,
-    # flow dt = 0.01 with plaquette / B every 10 steps (a short run here; the bench runs 100 steps)
-    from flue_lat_amd.ensemble import Ensemble
+def c4_dims():
+    # the batching workload: 32 configurations of 16^3 x Nt, Nt drawn (seeded) from {16,24,32,48,64,128}
     rng = np.random.default_rng(4000)
     nts = rng.choice([16, 24, 32, 48, 64, 128], size=32)
-    dims = [(int(nt), 16, 16, 16) for nt in nts]
+    return [(int(nt), 16, 16, 16) for nt in nts]
+
+
+def test_ensemble_c4_shape_fp32(FL):
+    # the batching workload in FP32 links: eps = 0.4, flow dt = 0.01 with plaquette / B every 10 steps,
+    # here 20 of the 100 steps bench.py's ensemble leg runs
+    from flue_lat_amd.ensemble import Ensemble
+    dims = c4_dims()
     e = Ensemble.random(dims, [4000 + j for j in range(32)], 0.4, "fp32")
     plaq, B = e.flow(0.01, 20, measure_every=10, use_graph=True)
     vols = np.array([np.prod(d) for d in dims])
@@ -85,3 +90,25 @@ def test_ensemble_c4_shape_fp32(FL):
         g = FL.GaugeField.random(dims[c], 4000 + c, 0.4, "fp64")
         g.wilson_flow(0.01, 20, use_graph=False)
         assert abs(B[-1, c] - g.magnetic()) < 1e-5 * abs(g.magnetic())
+
+
+def test_ensemble_c4_member_against_oracle_fp64(FL):
+    # the whole 32-configuration ensemble in FP64, two RK3 steps in one graph; the members
+    # with Nt = 16 and the first member of each other extent against the oracle's flow
+    from flue_lat_amd.ensemble import Ensemble
+    dims = c4_dims()
+    seeds = [4000 + j for j in range(32)]
+    e = Ensemble.random(dims, seeds, 0.4, "fp64")
+    plaq, B = e.flow(0.01, 2, measure_every=2, use_graph=True)
+    firsts = {}
+    for c, d in enumerate(dims):
+        firsts.setdefault(d[0], c)
+    checked = sorted(set(firsts.values()) | {c for c, d in enumerate(dims) if d[0] == 16})
+    for c in checked:
+        if dims[c][0] > 32:
+            continue   # the oracle's serial flow on Nt >= 48 takes too long for a test
+        print(f"[c4] member {c} {dims[c]} against the oracle", flush=True)
+        W = O.wilson_flow(O.random_field(dims[c], seeds[c], 0.4), 0.01, 2)
+        assert np.abs(e.field(c).to_complex().cpu().numpy() - W).max() < 1e-12, c
+        assert abs(B[1, c] - O.magnetic(W)) < 1e-12 * O.magnetic(W), c
+        assert abs(plaq[1, c] - O.plaquette(W, 1, 4, 4)[0]) < 1e-12 * abs(plaq[1, c]), c

@@ -28,10 +28,27 @@ def FL():
     return FL
 
 
+def dir_bytes(prec, k):
+    """bytes of the 9 complex planes of one direction of the tiled field"""
+    return 18 * SMALL[0] * k * VS * (8 if prec == "fp64" else 4)
+
+
 @pytest.mark.parametrize("prec,k", [("fp64", 920), ("fp32", 1830)])
 def test_unbuffered_view_matches_small_field(FL, prec, k):
-    bytes_per = 8 if prec == "fp64" else 4
-    assert 18 * SMALL[0] * k * VS * bytes_per >= 2 ** 32   # past the buffer-resource range
+    assert dir_bytes(prec, k) >= 2 ** 32   # past the buffer-resource range
+    check_tiled(FL, prec, k)
+
+
+@pytest.mark.parametrize("prec,k", [("fp64", 600), ("fp32", 1200)])
+def test_buffer_records_above_2gib(FL, prec, k):
+    # buffer resources whose record count has bit 31 set (2 GiB <= 9 planes < 4 GiB): the
+    # common 64^4 FP64 lattice is in this window
+    assert 2 ** 31 <= dir_bytes(prec, k) < 2 ** 32
+    check_tiled(FL, prec, k)
+
+
+def check_tiled(FL, prec, k):
+    print(f"[large] {prec} tiled x{k}: {dir_bytes(prec, k) / 2 ** 30:.2f} GiB per direction", flush=True)
     g = FL.GaugeField.random(SMALL, 77, 0.3, "fp64").astype(prec)
     big = tiled(FL, g, k)
     try:
@@ -56,8 +73,9 @@ def test_unbuffered_view_matches_small_field(FL, prec, k):
         torch.cuda.empty_cache()
 
 
-def test_unbuffered_flow_step_acts_slice_by_slice(FL):
-    k = 920
+@pytest.mark.parametrize("k", [920, 600])
+def test_unbuffered_flow_step_acts_slice_by_slice(FL, k):
+    # k = 920: past the buffer-resource range; k = 600: record count above 2 GiB
     g = FL.GaugeField.random(SMALL, 78, 0.3, "fp64")
     big = tiled(FL, g, k)
     try:
