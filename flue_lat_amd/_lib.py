@@ -79,6 +79,7 @@ SIGNATURES = {
     "flue_scalar_gluon_prop": [_P, _P, _I, _P, _P],
     "flue_magnetic": [_P, _P, _I, _P, _P, _SZ, _P],
     "flue_magnetic_ex": [_P, _P, _I, _I, _P, _P, _SZ, _P],
+    "flue_magnetic_noalloc": [_P, _P, _I, _I, _P, _P],
     "flue_b2_work_bytes": [_P, _I],
     "flue_flow_work_bytes": [_P, _I],
     "flue_wilson_action": [_P, _P, _I, _D, _P, _P, _SZ, _P],

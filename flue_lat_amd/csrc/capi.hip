@@ -602,6 +602,59 @@ int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t
     return FLUE_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free magnetic B
+
+}  // extern "C"
+
+// Scratch of flue_magnetic_noalloc: the per-block compensated partials of the magnetic
+// kernel and the tail of the two-stage finalize.  Each GPU holds its own copy at its own
+// address until the process ends.  Host code never uses this name as an address (on the
+// host it is only a stand-in for the device symbol); it uses the per-device address from
+// hipGetSymbolAddress instead.
+static __device__ __align__(256) unsigned char scratch[kMagneticScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+void* g_scratch_dev[64];  // device address of `scratch` per device, looked up on first use
+
+// device address of `scratch` on the current device, looked up once per device with that
+// device current
+int scratch_address(void** out) {
+    int dev = 0;
+    FLUE_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return FLUE_EINVAL;
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        FLUE_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return FLUE_EHIP;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return FLUE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flue_magnetic_noalloc(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B,
+                          void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (check_prec(prec) || variant < 0 || variant > kMagneticVariants) return FLUE_EINVAL;
+    // the partials of the largest form (three per block of 256 sites) must fit the scratch
+    const Lat L = make_lat(dims);
+    if (partial_bytes(magnetic_blocks(L, prec == FLUE_F32)) > kMagneticScratchBytes) return FLUE_EWS;
+    if (!U || !d_B) return FLUE_ENULL;
+    void* w;
+    if ((rc = scratch_address(&w))) return rc;
+    // every partial the finalize reads is written by the magnetic kernel first, so nothing
+    // needs clearing between calls
+    return flue_magnetic_ex(dims, U, prec, variant, d_B, w, kMagneticScratchBytes, stream);
+}
+
 // ---------------------------------------------------------------- gradients and flow
 
 size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec) {

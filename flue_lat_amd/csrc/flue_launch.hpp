@@ -66,6 +66,9 @@ constexpr uint32_t kFinalChunk = 2048;
 
 // number of magnetic kernel variants (forms x occupancy targets) the autotuner chooses from
 constexpr int kMagneticVariants = 6;
+// static device scratch of the allocation-free magnetic entry point: partials for lattices
+// of up to 44 M sites (48^3x96 needs 2 MiB)
+constexpr size_t kMagneticScratchBytes = (size_t)8 << 20;
 hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, double* partials, hipStream_t st,
                            int variant, uint32_t* npart);
 

@@ -278,6 +278,18 @@ int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_
 int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B, void* ws,
                      size_t ws_bytes, void* stream);
 
+/* flue_magnetic_ex without a workspace and without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The per-block partial sums
+ * live in a static device array of 8 MiB that each GPU holds until the process ends, enough
+ * for lattices of up to about 44 M sites (48^3x96 needs 2 MiB); larger lattices return
+ * FLUE_EWS before any launch.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (on different streams or host threads) race, so serialise them.  The
+ * kernels write every partial the reduction reads, so no clearing is needed between calls.
+ * Results are identical to flue_magnetic_ex with the same variant. */
+int flue_magnetic_noalloc(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B,
+                          void* stream);
+
 /* ------------------------------------------------------------------ */
 /* 3. gradients and Wilson flow (no counterpart in the reference)       */
 /* ------------------------------------------------------------------ */

@@ -1,0 +1,75 @@
+"""flue_magnetic_noalloc: the magnetic B with its partial sums in the library's static
+device scratch instead of a workspace, for callers that cannot allocate.  Called through
+the C ABI and checked against flue_magnetic_ex on the same field: the kernels and the
+fixed-order reduction are the same, so the results must agree bit for bit.  Two calls in a
+row, the second on a smaller lattice, show that the shared scratch carries nothing from
+one call to the next.
+"""
+import ctypes
+
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+EWS = 10
+
+
+@pytest.fixture(scope="module")
+def FL():
+    import flue_lat_amd as FL
+    return FL
+
+
+def noalloc(lib):
+    return lib.flue_magnetic_noalloc
+
+
+def run_both(FL, fn, g, variant):
+    from flue_lat_amd._lib import check
+    out = torch.empty(1, dtype=torch.float64, device=g.data.device)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(fn(g._cdims, ctypes.c_void_p(g.data.data_ptr()), g.prec, variant, ctypes.c_void_p(out.data_ptr()), st),
+          "flue_magnetic_noalloc")
+    got = out.item()
+    want = g.magnetic(variant=variant)
+    return got, want
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_noalloc_matches_workspace_entry(FL, prec):
+    from flue_lat_amd._lib import lib
+    fn = noalloc(lib())
+    # the larger lattice first, then a smaller one: the second call reads only what it wrote
+    for dims, seed in (((24, 16, 12, 20), 61), ((6, 5, 7, 4), 62)):
+        print(f"[noalloc] {prec} {dims}", flush=True)
+        U = O.random_field(dims, seed, 0.3)
+        g = FL.GaugeField.from_complex(U, prec)
+        for variant in (0, 1, 4, 5):
+            got, want = run_both(FL, fn, g, variant)
+            assert got == want, (dims, variant, got, want)
+        tol = 1e-12 if prec == "fp64" else 1e-6
+        assert abs(got - O.magnetic(U)) <= tol * O.magnetic(U)
+
+
+def test_noalloc_benchmark_lattice(FL):
+    from flue_lat_amd._lib import lib
+    fn = noalloc(lib())
+    g = FL.GaugeField.random((96, 48, 48, 48), 1003, 0.3).astype("fp32")
+    got, want = run_both(FL, fn, g, 0)
+    assert got == want
+    del g
+    torch.cuda.empty_cache()
+
+
+def test_noalloc_refuses_lattices_beyond_its_scratch(FL):
+    from flue_lat_amd._lib import lib
+    fn = noalloc(lib())
+    # 64^4 x 3 sites: more partials than the 8 MiB scratch holds; refused before any launch
+    dims = (ctypes.c_int32 * 4)(192, 64, 64, 64)
+    assert fn(dims, None, 0, 0, None, None) == EWS
+    dims = (ctypes.c_int32 * 4)(8, 8, 8, 8)
+    assert fn(dims, None, 0, 0, None, None) == 2      # FLUE_ENULL for the missing field
+    assert fn(dims, None, 0, 9, None, None) == 1      # FLUE_EINVAL for a variant out of range
