@@ -87,10 +87,15 @@ def cpu_rows(U, threads):
 
 def gpu_rows(fields, torch):
     """Event-timed kernels of the other operations on the benchmark's fields (ms per call,
-    mean of 5 after one warm-up; not part of the headline)."""
+    mean of 5 after one warm-up, 3 for the B^2 adjoint; not part of the headline).  The
+    autotuner first picks the plaquette walk length, the stencil visit order and the B^2
+    adjoint form on each field (variants chosen are in `tuned`); the flow step runs on a
+    copy, so the fields stay as measured."""
     V = DIMS[0] * DIMS[1] * DIMS[2] * DIMS[3]
     out = {}
     for name, g in fields:
+        tuned = {op: g.autotune(op, reps=2)[0] for op in ("plaquette", "flow", "b2_grad")}
+        flowed = g.clone()
         ops = {
             "plaquette": lambda: g.plaquette(1, 4, 4, sync=False),
             "polyakov": lambda: g.polyakov(sync=False),
@@ -98,11 +103,15 @@ def gpu_rows(fields, torch):
             "wilson_loops_6x6": lambda: g.wilson_loops(2, 1, 6, 6),
             "propagator": lambda: g.gluon_propagator(mu_start=1),
             "wilson_grad": lambda: g.wilson_grad(6.0),
+            "flow_step": lambda: flowed.wilson_flow(0.01, 1, use_graph=False),
+            "b2_grad": lambda: g.magnetic_grad(),
         }
-        r = {}
+        r = {"tuned": tuned}
         for op, fn in ops.items():
-            t_ev, _ = time_steps(fn, 5, 1, torch)
+            t_ev, _ = time_steps(fn, 3 if op == "b2_grad" else 5, 1, torch)
             r[op] = {"ms": round(t_ev * 1e3, 4), "gsites": round(V / t_ev / 1e9, 4)}
+        del flowed
+        torch.cuda.empty_cache()
         link = 72 if name == "fp32" else 144
         r["plaquette"]["hbm_frac"] = round(4 * link * V / (r["plaquette"]["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
         r["polyakov"]["hbm_frac"] = round(link * V / (r["polyakov"]["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
@@ -147,11 +156,13 @@ def ensemble_leg(FL, torch):
            "measure_every": 10}
     for prec in ("fp32", "fp64"):
         e = Ensemble.random(dims, seeds, 0.4, prec)
+        grid_variant = e.autotune(reps=2)[0]   # persistent grid, chosen on this ensemble
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         plaq, B = e.flow(0.01, 100, measure_every=10, use_graph=True)
         dt = time.perf_counter() - t0
         out[prec] = {"s": round(dt, 4), "gsite_steps_per_s": round(sites * 100 / dt / 1e9, 4),
+                     "grid_variant": grid_variant,
                      "mean_plaquette_final": float(np.mean(plaq[-1] / (18.0 * np.array([np.prod(d) for d in dims])))),
                      "mean_B_final": float(np.mean(B[-1]))}
         del e

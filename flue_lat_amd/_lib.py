@@ -19,6 +19,8 @@ STATUS = {
     6: "FLUE_EHIP", 7: "FLUE_ENOMEM", 8: "FLUE_ENOTIMPL", 9: "FLUE_ENODEV", 10: "FLUE_EWS", 11: "FLUE_ESU3",
 }
 FLUE_ESU3 = 11
+# tunable operations (flue_autotune)
+OPS = {"magnetic": 0, "b2_grad": 1, "plaquette": 2, "flow": 3, "ensemble_flow": 4}
 
 
 class FlueError(RuntimeError):
@@ -85,9 +87,11 @@ SIGNATURES = {
     "flue_wilson_action": [_P, _P, _I, _D, _P, _P, _SZ, _P],
     "flue_wilson_grad": [_P, _P, _I, _D, _P, _P],
     "flue_b2_grad": [_P, _P, _I, _P, _P, _SZ, _P],
+    "flue_b2_grad_ex": [_P, _P, _I, _I, _P, _P, _SZ, _P],
     "flue_flow_stage": [_P, _P, _I, _P, _P, _D, _I, _P],
     "flue_wilson_flow": [_P, _P, _I, _D, _I, _I, _P, _SZ, _P],
     "flue_autotune": [_I, _P, _P, _I, _I, _P, _P, _P, _SZ, _P],
+    "flue_autotune_work_bytes": [_I, _P, _I],
     "flue_tuned_variant": [_I, _P, _I, _P],
     "flue_tune_save": [ctypes.c_char_p],
     "flue_tune_load": [ctypes.c_char_p],
@@ -95,10 +99,11 @@ SIGNATURES = {
     "flue_ensemble_work_bytes": [_I, _P, _I],
     "flue_ensemble_random": [_I, _P, _P, _D, _P, _I, _P],
     "flue_ensemble_flow": [_I, _P, _P, _I, _D, _I, _I, _P, _P, _I, _P, _SZ, _P],
+    "flue_ensemble_autotune": [_I, _P, _P, _I, _I, _P, _P, _P, _SZ, _P],
 }
 _RESTYPE = {"flue_strerror": ctypes.c_char_p, "flue_field_bytes": _SZ, "flue_workspace_bytes": _SZ,
             "flue_b2_work_bytes": _SZ, "flue_flow_work_bytes": _SZ, "flue_ensemble_field_bytes": _SZ,
-            "flue_ensemble_work_bytes": _SZ}
+            "flue_ensemble_work_bytes": _SZ, "flue_autotune_work_bytes": _SZ}
 
 
 def lib():

@@ -90,6 +90,23 @@ size_t partial_bytes(uint32_t nblocks) {
 // per-block (count, max, max) triples and the combined triple
 size_t check_bytes(const Lat& L) { return ((size_t)check_blocks(L) * 3 + 3) * sizeof(double) + 256; }
 
+// B^2 adjoint work buffer: the anti-Hermitian clover sums (27 reals per site), 256-aligned
+size_t b2_ah_bytes(const Lat& L, int32_t prec) {
+    return ((size_t)27 * L.V * (prec == FLUE_F32 ? 4 : 8) + 255) & ~(size_t)255;
+}
+
+// the B^2 adjoint after the SU(3) check: clover sums into the start of `work`, then variant
+// 1 (per-link gather) or 2 (reverse mode with the slot buffer after the clover sums)
+int b2_grad_run(const Lat& L, const void* U, int32_t prec, int variant, void* work, void* grad, hipStream_t st) {
+    const int f32 = prec == FLUE_F32;
+    FLUE_TRY(launch_magnetic_ah(L, U, f32, work, st));
+    if (variant == 1)
+        FLUE_TRY(launch_b2_grad(L, U, f32, work, grad, st));
+    else
+        FLUE_TRY(launch_b2_grad_rev(L, U, f32, work, (char*)work + b2_ah_bytes(L, prec), grad, st));
+    return FLUE_OK;
+}
+
 // largest accepted max|U U^+ - 1| and |det U - 1|: a few hundred FP32 roundings of a unit
 // entry for FP32 links, far above FP64 rounding for FP64 links
 double su3_bound(int32_t prec) { return prec == FLUE_F32 ? 1e-5 : 1e-10; }
@@ -138,6 +155,24 @@ int tuned_variant(int op, const int32_t dims[4], int prec) {
     std::lock_guard<std::mutex> lk(g_tune_mutex);
     auto it = g_tune.find(tune_key(op, dims, prec));
     return it == g_tune.end() ? 0 : it->second.variant;
+}
+
+// number of variants of each tunable operation (FLUE_OP_*)
+int op_variants(int op) {
+    switch (op) {
+    case FLUE_OP_MAGNETIC: return kMagneticVariants;
+    case FLUE_OP_B2_GRAD: return kB2Variants;
+    case FLUE_OP_PLAQUETTE: return kPlaquetteVariants;
+    case FLUE_OP_FLOW: return kFlowVariants;
+    case FLUE_OP_ENSEMBLE_FLOW: return kEnsembleVariants;
+    default: return 0;
+    }
+}
+
+// the tuned variant of `op` for (device, prec, key), else `dflt`
+int variant_or(int op, const int32_t key[4], int prec, int dflt) {
+    const int v = tuned_variant(op, key, prec);
+    return v > 0 ? v : dflt;
 }
 
 hipStream_t S(void* s) { return (hipStream_t)s; }
@@ -376,7 +411,7 @@ size_t flue_field_bytes(const int32_t dims[4], int32_t prec) {
 size_t flue_workspace_bytes(const int32_t dims[4]) {
     if (check_dims(dims)) return 0;
     const Lat L = make_lat(dims);
-    uint32_t nb = plaquette_blocks(L);
+    uint32_t nb = plaquette_max_blocks(L);
     nb = std::max(nb, polyakov_blocks(L));
     nb = std::max(nb, path_blocks(L, -1, nullptr, 0));
     nb = std::max(nb, magnetic_blocks(L, 0));
@@ -439,10 +474,12 @@ int flue_plaquette(const int32_t dims[4], const void* U, int32_t prec, int32_t m
     if (!U || !d_sum) return FLUE_ENULL;
     if (check_prec(prec) || check_plaq_args(muStart, muEnd, nuEnd)) return FLUE_EINVAL;
     const Lat L = make_lat(dims);
-    const uint32_t nb = plaquette_blocks(L);
+    const int variant = variant_or(FLUE_OP_PLAQUETTE, dims, prec, kPlaquetteDefault);
+    uint32_t nb = plaquette_blocks(L, variant);
     void* w;
     if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
-    FLUE_TRY(launch_plaquette(L, U, prec == FLUE_F32, muStart - 1, muEnd - 1, nuEnd - 1, (double*)w, S(stream)));
+    FLUE_TRY(launch_plaquette(L, U, prec == FLUE_F32, muStart - 1, muEnd - 1, nuEnd - 1, (double*)w, S(stream),
+                              variant, &nb));
     FLUE_TRY(launch_finalize((double*)w, nb, d_sum, 1.0, S(stream)));
     return FLUE_OK;
 }
@@ -659,9 +696,10 @@ int flue_magnetic_noalloc(const int32_t dims[4], const void* U, int32_t prec, in
 
 size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec) {
     if (check_dims(dims) || check_prec(prec)) return 0;
-    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
-    // the per-site anti-Hermitian clover sums; the SU(3) check uses the same buffer first
-    return std::max((size_t)(27 * V * (prec == FLUE_F32 ? 4 : 8)), check_bytes(make_lat(dims)));
+    const Lat L = make_lat(dims);
+    // the per-site anti-Hermitian clover sums, then the slot buffer of the reverse form (one
+    // slab of time slices); the SU(3) check uses the start of the buffer first
+    return std::max(b2_ah_bytes(L, prec) + b2_slot_bytes(L, prec == FLUE_F32), check_bytes(L));
 }
 
 size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec) {
@@ -678,10 +716,11 @@ int flue_wilson_action(const int32_t dims[4], const void* U, int32_t prec, doubl
     if (!U || !d_S) return FLUE_ENULL;
     if (check_prec(prec) || !std::isfinite(beta)) return FLUE_EINVAL;
     const Lat L = make_lat(dims);
-    const uint32_t nb = plaquette_blocks(L);
+    const int variant = variant_or(FLUE_OP_PLAQUETTE, dims, prec, kPlaquetteDefault);
+    uint32_t nb = plaquette_blocks(L, variant);
     void* w;
     if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
-    FLUE_TRY(launch_plaquette(L, U, prec == FLUE_F32, 0, 3, 3, (double*)w, S(stream)));
+    FLUE_TRY(launch_plaquette(L, U, prec == FLUE_F32, 0, 3, 3, (double*)w, S(stream), variant, &nb));
     // S = beta * sum_p (1 - ReTr P / 3) over the 6V plaquettes
     FLUE_TRY(launch_finalize_affine((double*)w, nb, d_S, -beta / 3.0, beta * 6.0 * (double)L.V, S(stream)));
     return FLUE_OK;
@@ -692,7 +731,8 @@ int flue_wilson_grad(const int32_t dims[4], const void* U, int32_t prec, double 
     if (rc) return rc;
     if (!U || !grad) return FLUE_ENULL;
     if (check_prec(prec) || !std::isfinite(beta)) return FLUE_EINVAL;
-    return hip_status(launch_wilson_grad(make_lat(dims), U, prec == FLUE_F32, beta, grad, S(stream)));
+    return hip_status(launch_wilson_grad(make_lat(dims), U, prec == FLUE_F32, beta, grad, S(stream),
+                                         variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault)));
 }
 
 int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad, void* work, size_t work_bytes,
@@ -707,9 +747,21 @@ int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad,
     // the gather takes the product from a corner to the walk's end as X_c^+ F, which holds
     // for unitary links only: refuse fields that are not SU(3) to the precision's bound
     if ((rc = run_su3_check(L, U, prec, 0.0, nullptr, A, S(stream)))) return rc;
-    FLUE_TRY(launch_magnetic_ah(L, U, prec == FLUE_F32, A, S(stream)));
-    FLUE_TRY(launch_b2_grad(L, U, prec == FLUE_F32, A, grad, S(stream)));
-    return FLUE_OK;
+    return b2_grad_run(L, U, prec, variant_or(FLUE_OP_B2_GRAD, dims, prec, kB2DefaultVariant), A, grad, S(stream));
+}
+
+int flue_b2_grad_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, void* grad, void* work,
+                    size_t work_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U || !grad) return FLUE_ENULL;
+    if (check_prec(prec) || variant < 0 || variant > kB2Variants) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    void* A;
+    if ((rc = get_ws(work, work_bytes, flue_b2_work_bytes(dims, prec), &A))) return rc;
+    if ((rc = run_su3_check(L, U, prec, 0.0, nullptr, A, S(stream)))) return rc;
+    if (variant == 0) variant = variant_or(FLUE_OP_B2_GRAD, dims, prec, kB2DefaultVariant);
+    return b2_grad_run(L, U, prec, variant, A, grad, S(stream));
 }
 
 int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* Wout, double* acc, double eps,
@@ -723,7 +775,7 @@ int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* 
     static const double ca[3] = {0.0, -17.0 / 9.0, -1.0};
     static const double cb[3] = {0.25, 8.0 / 9.0, 0.75};
     return hip_status(launch_flow_stage(make_lat(dims), Win, prec == FLUE_F32, Wout, acc, eps, ca[stage], cb[stage],
-                                        stage == 2, S(stream)));
+                                        stage == 2, S(stream), variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault)));
 }
 
 int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, int32_t nsteps, int32_t use_graph,
@@ -792,56 +844,102 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
 
 // ---------------------------------------------------------------- autotuner
 
-int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec, int32_t reps, int32_t* best,
-                  double* best_ms, void* ws, size_t ws_bytes, void* stream) {
-    int rc = check_dims(dims);
-    if (rc) return rc;
-    if (!U) return FLUE_ENULL;
-    if (op != FLUE_OP_MAGNETIC || check_prec(prec) || reps < 1) return FLUE_EINVAL;
-    const Lat L = make_lat(dims);
-    const int is_f32 = prec == FLUE_F32, su3 = prec == FLUE_F64_SU3;
-    if (!fits_buffer(L.V, is_f32)) {
-        // only one variant exists for fields beyond the buffer-resource range
-        if (best) *best = magnetic_default_variant(is_f32);
-        if (best_ms) *best_ms = -1.0;
-        return FLUE_OK;
-    }
-    const uint32_t nb = magnetic_blocks(L, is_f32);
-    void* w;
-    if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
-    hipStream_t st = S(stream);
+}  // extern "C"
+
+namespace {
+
+// hipEvent timing of `reps` calls of launch(v) after one warm-up call, for v = 1..nvar
+template <typename Fn>
+int time_variants(int nvar, int reps, hipStream_t st, Fn&& launch, int* best, double* best_ms) {
     hipEvent_t e0, e1;
     FLUE_TRY(hipEventCreate(&e0));
     if (hipError_t ec = hipEventCreate(&e1); ec != hipSuccess) {
         (void)hipEventDestroy(e0);
         return hip_status(ec);
     }
-    int bv = 0;
+    int bv = 0, rc = FLUE_OK;
     float bms = 0.f;
     hipError_t e = hipSuccess;
-    for (int v = 1; v <= kMagneticVariants && e == hipSuccess; ++v) {
-        e = launch_magnetic(L, U, is_f32, su3, (double*)w, st, v, nullptr);  // warm-up
-        if (e == hipSuccess) e = hipEventRecord(e0, st);
-        for (int r = 0; r < reps && e == hipSuccess; ++r)
-            e = launch_magnetic(L, U, is_f32, su3, (double*)w, st, v, nullptr);
-        if (e == hipSuccess) e = hipEventRecord(e1, st);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
+    for (int v = 1; v <= nvar && e == hipSuccess && rc == FLUE_OK; ++v) {
+        rc = launch(v);  // warm-up
+        if (rc == FLUE_OK) e = hipEventRecord(e0, st);
+        for (int r = 0; r < reps && e == hipSuccess && rc == FLUE_OK; ++r) rc = launch(v);
+        if (rc == FLUE_OK && e == hipSuccess) e = hipEventRecord(e1, st);
+        if (rc == FLUE_OK && e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e == hipSuccess && (bv == 0 || ms < bms)) {
+        if (rc == FLUE_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (rc == FLUE_OK && e == hipSuccess && (bv == 0 || ms < bms)) {
             bv = v;
             bms = ms;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (rc != FLUE_OK) return rc;
     FLUE_TRY(e);
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mutex);
-        g_tune[tune_key(op, dims, prec)] = TuneVal{bv, (double)bms / reps};
+    *best = bv;
+    *best_ms = (double)bms / reps;
+    return FLUE_OK;
+}
+
+void remember(int op, const int32_t key[4], int prec, int v, double ms) {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    g_tune[tune_key(op, key, prec)] = TuneVal{v, ms};
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t flue_autotune_work_bytes(int32_t op, const int32_t dims[4], int32_t prec) {
+    if (check_dims(dims) || check_prec(prec)) return 0;
+    const Lat L = make_lat(dims);
+    const size_t fb = flue_field_bytes(dims, prec);
+    switch (op) {
+    case FLUE_OP_MAGNETIC: return partial_bytes(magnetic_blocks(L, prec == FLUE_F32));
+    case FLUE_OP_PLAQUETTE: return partial_bytes(plaquette_max_blocks(L));
+    case FLUE_OP_B2_GRAD: return flue_b2_work_bytes(dims, prec) + fb;  // work, then a gradient
+    case FLUE_OP_FLOW: return fb + (size_t)L.V * 32 * sizeof(double);  // output links, accumulator
+    default: return 0;
     }
+}
+
+int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec, int32_t reps, int32_t* best,
+                  double* best_ms, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_dims(dims);
+    if (rc) return rc;
+    if (!U) return FLUE_ENULL;
+    if (check_prec(prec) || reps < 1 || op_variants(op) == 0 || op == FLUE_OP_ENSEMBLE_FLOW) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    const int is_f32 = prec == FLUE_F32, su3 = prec == FLUE_F64_SU3;
+    if (op == FLUE_OP_MAGNETIC && !fits_buffer(L.V, is_f32)) {
+        // only one variant exists for fields beyond the buffer-resource range
+        if (best) *best = magnetic_default_variant(is_f32);
+        if (best_ms) *best_ms = -1.0;
+        return FLUE_OK;
+    }
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, flue_autotune_work_bytes(op, dims, prec), &w))) return rc;
+    hipStream_t st = S(stream);
+    const size_t fb = flue_field_bytes(dims, prec);
+    if (op == FLUE_OP_B2_GRAD && (rc = run_su3_check(L, U, prec, 0.0, nullptr, w, st))) return rc;
+    if (op == FLUE_OP_FLOW) FLUE_TRY(hipMemsetAsync((char*)w + fb, 0, (size_t)L.V * 32 * sizeof(double), st));
+    auto launch = [&](int v) -> int {
+        switch (op) {
+        case FLUE_OP_MAGNETIC: return hip_status(launch_magnetic(L, U, is_f32, su3, (double*)w, st, v, nullptr));
+        case FLUE_OP_PLAQUETTE: return hip_status(launch_plaquette(L, U, is_f32, 0, 3, 3, (double*)w, st, v, nullptr));
+        case FLUE_OP_B2_GRAD: return b2_grad_run(L, U, prec, v, w, (char*)w + flue_b2_work_bytes(dims, prec), st);
+        default:  // FLUE_OP_FLOW: the middle RK3 stage (it reads the accumulator), into scratch links
+            return hip_status(launch_flow_stage(L, U, is_f32, w, (double*)((char*)w + fb), 0.01, -17.0 / 9.0,
+                                                8.0 / 9.0, 0, st, v));
+        }
+    };
+    int bv = 0;
+    double bms = 0.0;
+    if ((rc = time_variants(op_variants(op), reps, st, launch, &bv, &bms))) return rc;
+    remember(op, dims, prec, bv, bms);
     if (best) *best = bv;
-    if (best_ms) *best_ms = (double)bms / reps;
+    if (best_ms) *best_ms = bms;
     return FLUE_OK;
 }
 
@@ -879,7 +977,7 @@ int flue_tune_load(const char* path) {
         TuneVal v;
         if (std::sscanf(line, "%d %d %d %d %d %d %d %d %lf", &k.dev, &k.op, &k.prec, &k.d[0], &k.d[1], &k.d[2],
                         &k.d[3], &v.variant, &v.ms) == 9 &&
-            v.variant >= 1 && v.variant <= kMagneticVariants)
+            v.variant >= 1 && v.variant <= op_variants(k.op))
             g_tune[k] = v;
     }
     std::fclose(f);
@@ -921,6 +1019,45 @@ int ens_plan(int32_t n, const int32_t* dims, EnsPlan& p) {
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// autotuner key of an ensemble: its size, logical blocks and largest volume
+void ens_key(const EnsPlan& p, int32_t n, int32_t key[4]) {
+    key[0] = n;
+    key[1] = (int32_t)p.total_blocks;
+    key[2] = (int32_t)p.maxV;
+    key[3] = 0;
+}
+
+// persistent grid of the ensemble kernels: 1, 2, 4 or 8 workgroups per CU (variant 1..4)
+int ens_grid(const EnsPlan& p, int variant, uint32_t* grid) {
+    int dev = 0, cus = 0;
+    FLUE_TRY(hipGetDevice(&dev));
+    FLUE_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t per_cu = 1u << (variant - 1);
+    *grid = std::max<uint32_t>(1, std::min<uint32_t>(p.total_blocks, (uint32_t)std::max(cus, 1) * per_cu));
+    return FLUE_OK;
+}
+
+// the ensemble work buffer: table, second link buffer, FP64 accumulators, partial sums
+struct EnsWork {
+    EnsEntry* tab;
+    void* W2;
+    double* acc;
+    double* partials;
+};
+
+EnsWork ens_work(void* w, int32_t n, const EnsPlan& p, size_t sz) {
+    char* base = (char*)w;
+    EnsWork r;
+    r.tab = (EnsEntry*)base;
+    base += align256(sizeof(EnsEntry) * n);
+    r.W2 = base;
+    base += align256((size_t)p.field_reals * sz);
+    r.acc = (double*)base;
+    base += align256((size_t)p.acc_doubles * 8);
+    r.partials = (double*)base;
+    return r;
+}
 
 }  // namespace
 
@@ -965,22 +1102,19 @@ int flue_ensemble_flow(int32_t n, const int32_t* dims, void* fields, int32_t pre
     const size_t sz = is_f32 ? 4 : 8;
     void* w;
     if ((rc = get_ws(work, work_bytes, flue_ensemble_work_bytes(n, dims, prec), &w))) return rc;
-    char* base = (char*)w;
-    EnsEntry* d_tab = (EnsEntry*)base;
-    base += align256(sizeof(EnsEntry) * n);
-    void* W2 = base;
-    base += align256((size_t)p.field_reals * sz);
-    double* acc = (double*)base;
-    base += align256((size_t)p.acc_doubles * 8);
-    double* partials = (double*)base;
+    const EnsWork ew = ens_work(w, n, p, sz);
+    EnsEntry* d_tab = ew.tab;
+    void* W2 = ew.W2;
+    double* acc = ew.acc;
+    double* partials = ew.partials;
     hipStream_t st = S(stream);
     FLUE_TRY(hipMemcpyAsync(d_tab, p.tab.data(), sizeof(EnsEntry) * n, hipMemcpyHostToDevice, st));
     FLUE_TRY(hipStreamSynchronize(st));  // the host table goes out of scope with this call
 
-    int dev = 0, cus = 0;
-    FLUE_TRY(hipGetDevice(&dev));
-    FLUE_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(p.total_blocks, (uint32_t)std::max(cus, 1) * 4));
+    int32_t key[4];
+    ens_key(p, n, key);
+    uint32_t grid = 0;
+    if ((rc = ens_grid(p, variant_or(FLUE_OP_ENSEMBLE_FLOW, key, prec, kEnsembleDefault), &grid))) return rc;
 
     const int nmeas = measure_every > 0 ? nsteps / measure_every + 1 : 0;
     static const double ca[3] = {0.0, -17.0 / 9.0, -1.0};
@@ -1032,6 +1166,41 @@ int flue_ensemble_flow(int32_t n, const int32_t* dims, void* fields, int32_t pre
     } else {
         FLUE_TRY(run());
     }
+    return FLUE_OK;
+}
+
+int flue_ensemble_autotune(int32_t n, const int32_t* dims, const void* fields, int32_t prec, int32_t reps,
+                           int32_t* best, double* best_ms, void* work, size_t work_bytes, void* stream) {
+    EnsPlan p;
+    int rc = ens_plan(n, dims, p);
+    if (rc) return rc;
+    if (!fields) return FLUE_ENULL;
+    if (check_prec(prec) || reps < 1) return FLUE_EINVAL;
+    const int is_f32 = prec == FLUE_F32;
+    const size_t sz = is_f32 ? 4 : 8;
+    void* w;
+    if ((rc = get_ws(work, work_bytes, flue_ensemble_work_bytes(n, dims, prec), &w))) return rc;
+    const EnsWork ew = ens_work(w, n, p, sz);
+    hipStream_t st = S(stream);
+    FLUE_TRY(hipMemcpyAsync(ew.tab, p.tab.data(), sizeof(EnsEntry) * n, hipMemcpyHostToDevice, st));
+    FLUE_TRY(hipMemsetAsync(ew.acc, 0, (size_t)p.acc_doubles * 8, st));
+    FLUE_TRY(hipStreamSynchronize(st));
+    // the middle RK3 stage of every field into the second buffer: the fields are not changed
+    auto launch = [&](int v) -> int {
+        uint32_t grid = 0;
+        int r = ens_grid(p, v, &grid);
+        if (r) return r;
+        return hip_status(launch_ens_flow_stage(ew.tab, n, p.total_blocks, p.maxV, fields, is_f32, ew.W2, ew.acc, 0.01,
+                                                -17.0 / 9.0, 8.0 / 9.0, 0, grid, st));
+    };
+    int bv = 0;
+    double bms = 0.0;
+    if ((rc = time_variants(kEnsembleVariants, reps, st, launch, &bv, &bms))) return rc;
+    int32_t key[4];
+    ens_key(p, n, key);
+    remember(FLUE_OP_ENSEMBLE_FLOW, key, prec, bv, bms);
+    if (best) *best = bv;
+    if (best_ms) *best_ms = bms;
     return FLUE_OK;
 }
 

@@ -24,9 +24,16 @@ hipError_t launch_check_field(const Lat& L, const void* U, int is_f32, double* p
 
 // reductions: each writes one compensated partial (hi, lo) per block to `partials`,
 // then launch_finalize sums them in a fixed order into out[0] (double, device)
-uint32_t plaquette_blocks(const Lat& L);
+// plaquette walk lengths (time slices per thread), one per variant; variant 0 = the default
+constexpr int kPlaquetteVariants = 6;
+constexpr int kPlaqWalks[kPlaquetteVariants] = {2, 3, 4, 6, 8, 12};
+constexpr int kPlaquetteDefault = 4;  // walks of 6 slices
+int plaquette_walk(int variant);
+uint32_t plaquette_blocks(const Lat& L, int variant);
+uint32_t plaquette_max_blocks(const Lat& L);
+// *npart (optional) receives the number of partials written
 hipError_t launch_plaquette(const Lat& L, const void* U, int is_f32, int mu0, int mu1, int nu1, double* partials,
-                            hipStream_t st);
+                            hipStream_t st, int variant, uint32_t* npart);
 uint32_t polyakov_blocks(const Lat& L);
 hipError_t launch_polyakov(const Lat& L, const void* U, int is_f32, double* partials, hipStream_t st);
 // blocks of path_trace over a slice (tslice >= 0) or the whole lattice (-1); with path ==
@@ -75,11 +82,26 @@ hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, dou
 // gradients and flow (k_grad.hip, k_b2grad.hip); grad / Wout / Abuf are SoA in the field
 // precision, acc is FP64 (8 su(3) coordinates x 4 directions x V)
 uint32_t grad_blocks(const Lat& L);
-hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st);
+// stencil visit orders (k_grad.hip): 1 = storage order, 2..6 = (z, t) bricks
+constexpr int kFlowVariants = 6;
+constexpr int kFlowDefault = 1;
+hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st,
+                              int variant);
 hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, double* acc, double eps,
-                             double ca, double cb, int reunit, hipStream_t st);
+                             double ca, double cb, int reunit, hipStream_t st, int variant);
 hipError_t launch_magnetic_ah(const Lat& L, const void* U, int is_f32, void* Abuf, hipStream_t st);
 hipError_t launch_b2_grad(const Lat& L, const void* U, int is_f32, const void* Abuf, void* grad, hipStream_t st);
+// reverse-mode B^2 adjoint: slab by slab of time slices, per-rectangle terms into `slots`
+// (b2_slot_bytes), then a fixed-order sum per link; slabs are sized so the slot buffer
+// stays within kB2SlotBudget (at least one slice)
+constexpr uint64_t kB2SlotBudget = (uint64_t)4 << 30;
+// B^2 adjoint forms: 1 = per-link gather, 2 = reverse mode (default)
+constexpr int kB2Variants = 2;
+constexpr int kB2DefaultVariant = 2;
+uint32_t b2_slab_slices(const Lat& L, int is_f32);
+size_t b2_slot_bytes(const Lat& L, int is_f32);
+hipError_t launch_b2_grad_rev(const Lat& L, const void* U, int is_f32, const void* Abuf, void* slots, void* grad,
+                              hipStream_t st);
 hipError_t launch_finalize_affine(const double* partials, uint32_t n, double* out, double scale, double offset,
                                   hipStream_t st);
 
@@ -101,5 +123,8 @@ hipError_t launch_ens_sum(const EnsEntry* tab, int n, uint32_t total, uint32_t m
 hipError_t launch_ens_finalize(const EnsEntry* tab, int n, const double* partials, double* out, int stride,
                                int per_volume, hipStream_t st);
 hipError_t launch_copy(size_t n_reals, int is_f32, const void* src, void* dst, hipStream_t st);
+// persistent grid of the ensemble kernels: 1, 2, 4 or 8 workgroups per CU
+constexpr int kEnsembleVariants = 4;
+constexpr int kEnsembleDefault = 3;
 
 }  // namespace flue

@@ -8,12 +8,24 @@
 //
 // The leaves through U are enumerated as rectangles: each a x b rectangle of the
 // 5-loop set that contains U is a leaf of each of its four corners (one per quadrant,
-// all with the weight k_ab).  Walking the rectangle once from U's far end with the
-// running product X (from the walk start to the current vertex) gives all four corner
-// insertions at once: sum_c X_c A(c) X_c^+ F with F the walk's full product (links are
-// unitary, so the product from a corner to the walk's end is X_c^+ F).  The clover
-// leaves are recomputed per site in a first pass that stores only A_i(x).
+// all with the weight k_ab).  The clover leaves are recomputed per site in a first pass
+// that stores only A_i(x).  Two forms then build the gradient:
+//
+// gather (variant 1): every link walks each rectangle through it from its far end with the
+// running product X (from the walk start to the current vertex), which gives all four
+// corner insertions at once: sum_c X_c A(c) X_c^+ F with F the walk's full product (links
+// are unitary, so the product from a corner to the walk's end is X_c^+ F).  A rectangle
+// of perimeter n is walked by each of its n links: n(n + 9) products per rectangle.
+//
+// reverse (variant 2, the default): every rectangle is walked once, from its lower-left
+// corner x0, and hands each of its n links its term; see b2_rect_kernel.  About 4n + 7
+// products per rectangle.  The terms go to fixed slots of a buffer, one per (plane,
+// rectangle shape, step of the loop) and base site, and a second pass adds the 52 slots of
+// each link in a fixed order, so the result is deterministic without atomics.  Rectangles
+// lie in spatial planes, so time slices are independent: the slot buffer covers a slab of
+// whole slices and the two passes run slab by slab.
 #include "flue_clover.hpp"
+#include "flue_flow.hpp"
 #include "flue_launch.hpp"
 #include "flue_su3x.hpp"
 
@@ -25,6 +37,14 @@ constexpr int kBS = 256;
 // magnetic planes (mu, nu) in 0-based directions: (y,x), (x,z), (z,y)  (src/wloops.F90:502-506)
 __constant__ int kPlaneMu[3] = {2, 1, 3};
 __constant__ int kPlaneNu[3] = {1, 3, 2};
+
+template <int A_, int B_>
+struct AB {
+    static constexpr int a = A_, b = B_;
+};
+
+// c[d] for a spatial direction d known only at run time (selects, so c stays in registers)
+__device__ __forceinline__ int coord_of(const int c[4], int d) { return d == 1 ? c[1] : (d == 2 ? c[2] : c[3]); }
 
 // A_i(x) stored as 9 planes per magnetic plane: d0 d1 d2, o01r o02r o12r, o01i o02i o12i
 template <typename FV>
@@ -239,6 +259,281 @@ __global__ void __launch_bounds__(kBS) b2_grad_kernel(Lat L, FV f, const typenam
     }
 }
 
+// ---------------------------------------------------------------- reverse mode
+//
+// Rectangle a x b in plane (mu, nu) with lower-left corner x0: its counter-clockwise loop
+// has vertices v_k = P(i, j) = x0 + i mu + j nu, links L_k (k = 0 .. n-1, n = 2(a + b):
+// a steps +mu, b steps +nu, a steps -mu, b steps -nu), prefixes X_k = L_0 ... L_{k-1} and
+// the loop W0 = X_n.  Its corners sit at steps 0, a, a+b, 2a+b; with the corner sums
+// transported to x0, Ac~ = X_c A(v_c) X_c^+, the leaves of the rectangle at its four
+// corners give the link of step k
+//     forward step  (L_k = U):    M = X_k^+ Z X_k
+//     backward step (L_k = U^+):  M = -X_{k+1}^+ Z X_{k+1}
+// with Z = (sum of Ac~ over the corners after step k) W0 + W0 (sum over the corners at or
+// before it): one matrix per side of the rectangle.  (The leaf at corner c through link k
+// is P(v_{k+1} -> c) A(c) P(c -> v_k) after the link; writing the paths with prefixes and
+// using unitarity gives the forms above.)  The prefixes are the corner chains of the
+// forward kernel: Line(mu, p) and CA(a, q) = Line(mu, a) Line(v_a, nu, q) on the bottom and
+// right sides, and for the top and left sides X_{k+1} = W0 CB(b, m) or W0 Line(nu, m) with
+// CB(b, m) = Line(nu, b) Line(v_{2a+b}, mu, m), so with Z' = W0^+ Z W0 they read
+// M = -Y^+ Z' Y for the CB-side chain Y.  Per side:
+//     bottom  Z  = (Aa~ + Aab~ + A2ab~) W0 + W0 A0
+//     right   Z  = (Aab~ + A2ab~) W0 + W0 (A0 + Aa~)
+//     top     Z' = (Ahat + A0 + Aa~ + Aab~) W0,       Ahat = Line(nu, b) A(v_{2a+b}) Line(nu, b)^+
+//     left    Z' = (A0 + Aa~ + Aab~ + A2ab~) W0,      A2ab~ = W0 Ahat W0^+
+
+constexpr int kRectSlots = 52;  // loop steps of the seven shapes: 4+8+12+6+6+8+8
+
+// shapes in slot order, with the slot of their step 0 and their weight
+// (k1, k2, k5 for 1x1, 2x2, 3x3; k3/2 for 1x2, 2x1; k4/2 for 1x3, 3x1; src/wloops.F90:219-224)
+template <int A, int B>
+struct RectShape;
+template <> struct RectShape<1, 1> { static constexpr int off = 0;  static constexpr double w = kK1; };
+template <> struct RectShape<2, 2> { static constexpr int off = 4;  static constexpr double w = kK2; };
+template <> struct RectShape<3, 3> { static constexpr int off = 12; static constexpr double w = kK5; };
+template <> struct RectShape<1, 2> { static constexpr int off = 24; static constexpr double w = 0.5 * kK3; };
+template <> struct RectShape<2, 1> { static constexpr int off = 30; static constexpr double w = 0.5 * kK3; };
+template <> struct RectShape<1, 3> { static constexpr int off = 36; static constexpr double w = 0.5 * kK4; };
+template <> struct RectShape<3, 1> { static constexpr int off = 44; static constexpr double w = 0.5 * kK4; };
+
+// X A X^+
+template <typename R>
+__device__ __forceinline__ Mat<R> conj_by(const Mat<R>& X, const Mat<R>& A) {
+    return mul_nd(mul(X, A), X);
+}
+
+template <typename R>
+__device__ __forceinline__ Mat<R> madd(const Mat<R>& a, const Mat<R>& b) {
+    Mat<R> c;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        c.r[e] = a.r[e] + b.r[e];
+        c.i[e] = a.i[e] + b.i[e];
+    }
+    return c;
+}
+
+// all three rows of a chain that may carry two (C2)
+template <bool C2, typename R>
+__device__ __forceinline__ Mat<R> full_rows(const Mat<R>& M) {
+    Mat<R> F = M;
+    if constexpr (C2) fill_row2(F);
+    return F;
+}
+
+// chain step M * U (two rows with C2)
+template <bool C2, typename R>
+__device__ __forceinline__ Mat<R> chain(const Mat<R>& M, const Mat<R>& U) {
+    if constexpr (C2)
+        return mul2(M, U);
+    else
+        return mul(M, U);
+}
+
+// slot k of this base: w * (su(3) coordinates of P_TA(X^+ Z X)), coordinates as su3_to8
+template <typename R>
+__device__ __forceinline__ void emit_slot(R* __restrict__ slots, uint32_t nsl, uint32_t o, int slot, R w,
+                                          const Mat<R>& M) {
+    const R tr = (M.i[0] + M.i[4] + M.i[8]) * R(1.0 / 3.0);
+    R v[8];
+    v[0] = M.i[0] - tr;
+    v[1] = M.i[4] - tr;
+    v[2] = R(0.5) * (M.r[1] - M.r[3]);
+    v[3] = R(0.5) * (M.i[1] + M.i[3]);
+    v[4] = R(0.5) * (M.r[2] - M.r[6]);
+    v[5] = R(0.5) * (M.i[2] + M.i[6]);
+    v[6] = R(0.5) * (M.r[5] - M.r[7]);
+    v[7] = R(0.5) * (M.i[5] + M.i[7]);
+    R* p = slots + (size_t)slot * 8 * nsl + o;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) p[(size_t)c * nsl] = w * v[c];
+}
+
+// X^+ Z X with X given by two or three rows
+template <bool C2, typename R>
+__device__ __forceinline__ Mat<R> sandwich(const Mat<R>& X, const Mat<R>& Z) {
+    const Mat<R> F = full_rows<C2>(X);
+    return mul(mul_dn(F, Z), F);
+}
+
+// One rectangle shape A x B for every base site of the slab [s0, s0 + nsl) and the
+// magnetic plane blockIdx.y: writes its n = 2(A + B) slots (plane, shape, step).
+template <int A, int B, typename FV>
+__global__ void __launch_bounds__(kBS) b2_rect_kernel(Lat L, FV f, const typename FV::real* __restrict__ Abuf,
+                                                      uint32_t s0, uint32_t nsl, typename FV::real* __restrict__ slots) {
+    using R = typename FV::real;
+    constexpr bool C2 = sizeof(R) == 4;  // FP32: the chains carry two rows (SU(3) links)
+    constexpr int OFF = RectShape<A, B>::off;
+    const R w = R(RectShape<A, B>::w);
+    const int pl = (int)blockIdx.y;
+    const uint32_t o = xcd_block(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    if (o >= nsl) return;
+    const int mu = pl == 0 ? 2 : (pl == 1 ? 1 : 3);
+    const int nu = pl == 0 ? 1 : (pl == 1 ? 3 : 2);
+    const uint32_t x0 = s0 + o;
+    int c[4];
+    site_coords(L, x0, c);
+    const uint32_t smu = L.stride(mu), snu = L.stride(nu);
+    const int cmu = mu == 1 ? c[1] : (mu == 2 ? c[2] : c[3]);
+    const int cnu = nu == 1 ? c[1] : (nu == 2 ? c[2] : c[3]);
+    const uint32_t sp = x0 - (uint32_t)cmu * smu - (uint32_t)cnu * snu;
+    uint32_t Tm[4], Tn[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        Tm[k] = (uint32_t)wrapc(cmu + k, L.ext(mu)) * smu;
+        Tn[k] = (uint32_t)wrapc(cnu + k, L.ext(nu)) * snu;
+    }
+#define P_(i, j) (sp + Tm[i] + Tn[j])
+    R* __restrict__ out = slots + (size_t)(pl * kRectSlots + OFF) * 8 * nsl;
+
+    // ---- the loop and its corner sums
+    Mat<R> Aa, Aab, Ahat, W0;
+    {
+        Mat<R> X = load_link(f, mu, P_(0, 0));  // Line(mu, 1)
+#pragma unroll
+        for (int p = 1; p < A; ++p) X = chain<C2>(X, load_link(f, mu, P_(p, 0)));
+        Aa = conj_by(full_rows<C2>(X), load_ah(Abuf, L.V, pl, P_(A, 0)));
+#pragma unroll
+        for (int q = 0; q < B; ++q) X = chain<C2>(X, load_link(f, nu, P_(A, q)));  // CA(A, B)
+        const Mat<R> CA = full_rows<C2>(X);
+        Aab = conj_by(CA, load_ah(Abuf, L.V, pl, P_(A, B)));
+        Mat<R> Y = load_link(f, nu, P_(0, 0));  // Line(nu, 1)
+#pragma unroll
+        for (int q = 1; q < B; ++q) Y = chain<C2>(Y, load_link(f, nu, P_(0, q)));
+        Ahat = conj_by(full_rows<C2>(Y), load_ah(Abuf, L.V, pl, P_(0, B)));
+#pragma unroll
+        for (int p = 0; p < A; ++p) Y = chain<C2>(Y, load_link(f, mu, P_(p, B)));  // CB(B, A)
+        W0 = mul_nd(CA, full_rows<C2>(Y));
+    }
+    const Mat<R> A0 = load_ah(Abuf, L.V, pl, P_(0, 0));
+    const Mat<R> A2ab = conj_by(W0, Ahat);
+    const Mat<R> T3 = madd(madd(A0, Aa), Aab);
+    const Mat<R> Zt = mul(madd(Ahat, T3), W0);                // top, W0-conjugated
+    const Mat<R> Zl = mul(madd(T3, A2ab), W0);                // left, W0-conjugated
+    const Mat<R> Sab = madd(Aab, A2ab);
+    const Mat<R> Zr = madd(mul(Sab, W0), mul(W0, madd(A0, Aa)));   // right
+    const Mat<R> Zb = madd(mul(madd(Aa, Sab), W0), mul(W0, A0));   // bottom
+
+    // ---- bottom (steps 0 .. A-1, X = Line(mu, p)) and right (steps A .. A+B-1, X = CA(A, q))
+    emit_slot(out, nsl, o, 0, w, Zb);
+    {
+        Mat<R> X = load_link(f, mu, P_(0, 0));
+#pragma unroll
+        for (int p = 1; p < A; ++p) {
+            emit_slot(out, nsl, o, p, w, sandwich<C2>(X, Zb));
+            X = chain<C2>(X, load_link(f, mu, P_(p, 0)));
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            emit_slot(out, nsl, o, A + q, w, sandwich<C2>(X, Zr));
+            if (q + 1 < B) X = chain<C2>(X, load_link(f, nu, P_(A, q)));
+        }
+    }
+    // ---- left (step 2A+B+j ends at P(0, B-1-j): Y = Line(nu, B-1-j)) and top (step A+B+j
+    // ends at P(A-1-j, B): Y = CB(B, A-1-j)), both backward, walked with the chain ascending
+    emit_slot(out, nsl, o, 2 * A + 2 * B - 1, -w, Zl);
+    {
+        Mat<R> Y = load_link(f, nu, P_(0, 0));
+#pragma unroll
+        for (int m = 1; m < B; ++m) {
+            emit_slot(out, nsl, o, 2 * A + 2 * B - 1 - m, -w, sandwich<C2>(Y, Zl));
+            Y = chain<C2>(Y, load_link(f, nu, P_(0, m)));
+        }
+#pragma unroll
+        for (int m = 0; m < A; ++m) {
+            emit_slot(out, nsl, o, 2 * A + B - 1 - m, -w, sandwich<C2>(Y, Zt));
+            if (m + 1 < A) Y = chain<C2>(Y, load_link(f, mu, P_(m, B)));
+        }
+    }
+#undef P_
+}
+
+// Second pass: every link of the slab adds its 52 slots (26 in each of the two magnetic
+// planes through its direction) in a fixed order; temporal links get 0.
+// blockIdx.y is the direction.
+template <typename R>
+__global__ void __launch_bounds__(kBS) b2_slot_sum_kernel(Lat L, const R* __restrict__ slots, uint32_t s0,
+                                                          uint32_t nsl, double scale, R* __restrict__ grad) {
+    const int d = (int)blockIdx.y;
+    const uint32_t o = xcd_block(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    if (o >= nsl) return;
+    const uint32_t x = s0 + o;
+    Mat<R> G;
+    if (d == 0) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) G.r[e] = G.i[e] = R(0);
+        store_link(grad, L.V, 0, x, G);
+        return;
+    }
+    int c[4];
+    site_coords(L, x, c);
+    double acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+    // slab-relative base site of the rectangle whose step `slot` is this link
+    auto add = [&](int pl, int slot, uint32_t base) {
+        const R* p = slots + ((size_t)(pl * kRectSlots + slot) * 8) * nsl + (base - s0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] += (double)p[(size_t)k * nsl];
+    };
+    // plane where d is the first direction (d = mu): bottom and top sides
+    {
+        const int pl = d == 2 ? 0 : (d == 1 ? 1 : 2);
+        const int e = pl == 0 ? 1 : (pl == 1 ? 3 : 2);  // that plane's nu
+        const uint32_t sd = L.stride(d), se = L.stride(e);
+        const int cd = coord_of(c, d), ce = coord_of(c, e);
+        const uint32_t sp = x - (uint32_t)cd * sd - (uint32_t)ce * se;
+        uint32_t Td[4], Te[4];  // sites at -k along d and e
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            Td[k] = (uint32_t)wrapc(cd - k, L.ext(d)) * sd;
+            Te[k] = (uint32_t)wrapc(ce - k, L.ext(e)) * se;
+        }
+        auto shape = [&](auto ab) {
+            constexpr int A = decltype(ab)::a, B = decltype(ab)::b, OFF = RectShape<A, B>::off;
+#pragma unroll
+            for (int p = 0; p < A; ++p) add(pl, OFF + p, sp + Td[p] + Te[0]);              // bottom, base x - p d
+#pragma unroll
+            for (int j = 0; j < A; ++j) add(pl, OFF + A + B + j, sp + Td[A - 1 - j] + Te[B]);  // top
+        };
+        shape(AB<1, 1>()); shape(AB<2, 2>()); shape(AB<3, 3>()); shape(AB<1, 2>());
+        shape(AB<2, 1>()); shape(AB<1, 3>()); shape(AB<3, 1>());
+    }
+    // plane where d is the second direction (d = nu): right and left sides
+    {
+        const int pl = d == 1 ? 0 : (d == 3 ? 1 : 2);
+        const int e = pl == 0 ? 2 : (pl == 1 ? 1 : 3);  // that plane's mu
+        const uint32_t sd = L.stride(d), se = L.stride(e);
+        const int cd = coord_of(c, d), ce = coord_of(c, e);
+        const uint32_t sp = x - (uint32_t)cd * sd - (uint32_t)ce * se;
+        uint32_t Td[4], Te[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            Td[k] = (uint32_t)wrapc(cd - k, L.ext(d)) * sd;
+            Te[k] = (uint32_t)wrapc(ce - k, L.ext(e)) * se;
+        }
+        auto shape = [&](auto ab) {
+            constexpr int A = decltype(ab)::a, B = decltype(ab)::b, OFF = RectShape<A, B>::off;
+#pragma unroll
+            for (int q = 0; q < B; ++q) add(pl, OFF + A + q, sp + Te[A] + Td[q]);                  // right
+#pragma unroll
+            for (int j = 0; j < B; ++j) add(pl, OFF + 2 * A + B + j, sp + Te[0] + Td[B - 1 - j]);  // left
+        };
+        shape(AB<1, 1>()); shape(AB<2, 2>()); shape(AB<3, 3>()); shape(AB<1, 2>());
+        shape(AB<2, 1>()); shape(AB<1, 3>()); shape(AB<3, 1>());
+    }
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = acc[k] * scale;
+    const Mat<double> Gd = su3_from8(v);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        G.r[e] = (R)Gd.r[e];
+        G.i[e] = (R)Gd.i[e];
+    }
+    store_link(grad, L.V, d, x, G);
+}
+
 }  // namespace
 
 hipError_t launch_magnetic_ah(const Lat& L, const void* U, int is_f32, void* Abuf, hipStream_t st) {
@@ -248,6 +543,46 @@ hipError_t launch_magnetic_ah(const Lat& L, const void* U, int is_f32, void* Abu
         magnetic_ah_kernel<F><<<g, kBS, 0, st>>>(L, f, (typename F::real*)Abuf);
     });
     return hipGetLastError();
+}
+
+uint32_t b2_slab_slices(const Lat& L, int is_f32) {
+    const uint64_t per_slice = (uint64_t)L.Vs * 3 * kRectSlots * 8 * (is_f32 ? 4 : 8);
+    uint64_t S = kB2SlotBudget / per_slice;
+    if (S < 1) S = 1;
+    if (S > (uint64_t)L.n[0]) S = (uint64_t)L.n[0];
+    return (uint32_t)S;
+}
+
+size_t b2_slot_bytes(const Lat& L, int is_f32) {
+    return (size_t)b2_slab_slices(L, is_f32) * L.Vs * 3 * kRectSlots * 8 * (is_f32 ? 4 : 8);
+}
+
+hipError_t launch_b2_grad_rev(const Lat& L, const void* U, int is_f32, const void* Abuf, void* slots, void* grad,
+                              hipStream_t st) {
+    const double scale = 1.0 / (16.0 * (double)L.V);
+    const uint32_t S = b2_slab_slices(L, is_f32);
+    hipError_t e = hipSuccess;
+    with_view(L.V, U, is_f32, [&](auto f) {
+        using F = decltype(f);
+        using R = typename F::real;
+        const R* A = (const R*)Abuf;
+        R* sl = (R*)slots;
+        for (uint32_t t0 = 0; t0 < (uint32_t)L.n[0] && e == hipSuccess; t0 += S) {
+            const uint32_t ns = std::min<uint32_t>(S, (uint32_t)L.n[0] - t0);
+            const uint32_t s0 = t0 * L.Vs, nsl = ns * L.Vs;
+            const dim3 g1((nsl + kBS - 1) / kBS, 3), g2((nsl + kBS - 1) / kBS, 4);
+            b2_rect_kernel<1, 1, F><<<g1, kBS, 0, st>>>(L, f, A, s0, nsl, sl);
+            b2_rect_kernel<2, 2, F><<<g1, kBS, 0, st>>>(L, f, A, s0, nsl, sl);
+            b2_rect_kernel<3, 3, F><<<g1, kBS, 0, st>>>(L, f, A, s0, nsl, sl);
+            b2_rect_kernel<1, 2, F><<<g1, kBS, 0, st>>>(L, f, A, s0, nsl, sl);
+            b2_rect_kernel<2, 1, F><<<g1, kBS, 0, st>>>(L, f, A, s0, nsl, sl);
+            b2_rect_kernel<1, 3, F><<<g1, kBS, 0, st>>>(L, f, A, s0, nsl, sl);
+            b2_rect_kernel<3, 1, F><<<g1, kBS, 0, st>>>(L, f, A, s0, nsl, sl);
+            b2_slot_sum_kernel<R><<<g2, kBS, 0, st>>>(L, sl, s0, nsl, scale, (R*)grad);
+            e = hipGetLastError();
+        }
+    });
+    return e;
 }
 
 hipError_t launch_b2_grad(const Lat& L, const void* U, int is_f32, const void* Abuf, void* grad, hipStream_t st) {

@@ -15,16 +15,37 @@ namespace {
 
 constexpr int kBS = 256;
 
+// Visit orders of the 256-site blocks (the autotuner's variants of the flow and gradient
+// stencils).  Every order runs the same per-link code, so results are bit-identical.
+//   bz = 0:  storage order (x, y, z, t).
+//   bz > 0:  bricks of bz z-planes x bt t-slices (whole (x, y) planes of 256k sites):
+//            within a brick t runs fastest, then z, then the blocks of a plane, so the
+//            blocks in flight on an XCD hold the z and t neighbours of what they read.
+struct BlockOrder {
+    uint32_t bz, bt;    // brick extents (0 = storage order)
+    uint32_t nbp;       // 256-site blocks per (z, t) plane
+    uint32_t ntb;       // bricks along t
+};
+
+__device__ __forceinline__ uint32_t site_block(const Lat& L, const BlockOrder& bo, uint32_t lb) {
+    if (bo.bz == 0) return lb;
+    const uint32_t k = lb % bo.nbp, q = lb / bo.nbp;
+    const uint32_t per = bo.bz * bo.bt, pib = q % per, b = q / per;
+    const uint32_t t = (b % bo.ntb) * bo.bt + pib % bo.bt;
+    const uint32_t z = (b / bo.ntb) * bo.bz + pib / bo.bt;
+    return (t * (uint32_t)L.n[3] + z) * bo.nbp + k;
+}
+
 // One thread per link.  The direction is the fastest block index: the four blocks of the
 // same 256 sites run together and read the same neighbour links, so they share them in L2
 // (a thread walking the four directions of its site read them again long after).
 template <typename FV>
-__global__ void __launch_bounds__(kBS) wilson_grad_kernel(Lat L, FV f, double beta,
+__global__ void __launch_bounds__(kBS) wilson_grad_kernel(Lat L, FV f, BlockOrder bo, double beta,
                                                           typename FV::real* __restrict__ grad) {
     using R = typename FV::real;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int mu = (int)(blk & 3u);
-    const uint32_t s = (blk >> 2) * kBS + threadIdx.x;
+    const uint32_t s = site_block(L, bo, blk >> 2) * kBS + threadIdx.x;
     if (s >= L.V) return;
     int c[4];
     site_coords(L, s, c);
@@ -42,12 +63,12 @@ __global__ void __launch_bounds__(kBS) wilson_grad_kernel(Lat L, FV f, double be
 }
 
 template <typename FV>
-__global__ void __launch_bounds__(kBS) flow_stage_kernel(Lat L, FV f, typename FV::real* __restrict__ wout,
+__global__ void __launch_bounds__(kBS) flow_stage_kernel(Lat L, FV f, BlockOrder bo, typename FV::real* __restrict__ wout,
                                                          double* __restrict__ acc, double eps, double ca, double cb,
                                                          int reunit) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int mu = (int)(blk & 3u);
-    const uint32_t s = (blk >> 2) * kBS + threadIdx.x;
+    const uint32_t s = site_block(L, bo, blk >> 2) * kBS + threadIdx.x;
     if (s >= L.V) return;
     int c[4];
     site_coords(L, s, c);
@@ -56,24 +77,42 @@ __global__ void __launch_bounds__(kBS) flow_stage_kernel(Lat L, FV f, typename F
 
 }  // namespace
 
+// brick extents (z-planes, t-slices) of the stencil variants; variant 1 is the storage order
+constexpr uint32_t kBrickZ[kFlowVariants] = {0, 1, 2, 1, 2, 4};
+constexpr uint32_t kBrickT[kFlowVariants] = {0, 2, 2, 4, 4, 4};
+
+// the order of a variant, or the storage order where the bricks do not tile the lattice
+// (planes of a whole number of blocks, extents divisible by the brick)
+static BlockOrder block_order(const Lat& L, int variant) {
+    BlockOrder bo{0, 0, 1, 1};
+    if (variant < 1 || variant > kFlowVariants) variant = kFlowDefault;
+    const uint32_t bz = kBrickZ[variant - 1], bt = kBrickT[variant - 1];
+    const uint32_t plane = (uint32_t)L.n[1] * (uint32_t)L.n[2];
+    if (bz == 0 || plane % kBS != 0 || L.n[3] % bz != 0 || L.n[0] % bt != 0) return bo;
+    return BlockOrder{bz, bt, plane / kBS, (uint32_t)L.n[0] / bt};
+}
+
 // four blocks (one per direction) per 256 sites
 uint32_t grad_blocks(const Lat& L) { return 4u * ((L.V + kBS - 1) / kBS); }
 
-hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st) {
+hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st,
+                              int variant) {
     const uint32_t g = grad_blocks(L);
+    const BlockOrder bo = block_order(L, variant);
     with_view(L.V, U, is_f32, [&](auto f) {
         using F = decltype(f);
-        wilson_grad_kernel<F><<<g, kBS, 0, st>>>(L, f, beta, (typename F::real*)grad);
+        wilson_grad_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, beta, (typename F::real*)grad);
     });
     return hipGetLastError();
 }
 
 hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, double* acc, double eps,
-                             double ca, double cb, int reunit, hipStream_t st) {
+                             double ca, double cb, int reunit, hipStream_t st, int variant) {
     const uint32_t g = grad_blocks(L);
+    const BlockOrder bo = block_order(L, variant);
     with_view(L.V, Win, is_f32, [&](auto f) {
         using F = decltype(f);
-        flow_stage_kernel<F><<<g, kBS, 0, st>>>(L, f, (typename F::real*)Wout, acc, eps, ca, cb, reunit);
+        flow_stage_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, (typename F::real*)Wout, acc, eps, ca, cb, reunit);
     });
     return hipGetLastError();
 }

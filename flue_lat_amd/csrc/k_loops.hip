@@ -1,6 +1,8 @@
 // k_loops.hip -- plaquette family, Polyakov loop, generic-path products and the
 // deterministic second pass of every lattice sum.
 #include "flue_flow.hpp"
+#include <algorithm>
+
 #include "flue_launch.hpp"
 
 namespace flue {
@@ -33,11 +35,10 @@ __device__ __forceinline__ void write_partial(double* partials, uint32_t blk, DD
 // at x + t, which the temporal planes (0, nu) read as neighbours, are kept in registers as
 // the own links of the next slice, so each site reads its own links once and, per slice,
 // 1 temporal link, 3 links at x + t and 9 links of spatial neighbours (against 21 link
-// reads for the planes taken one by one).
-constexpr int kPlaqLT = 6;
-
+// reads for the planes taken one by one).  The walk length lt is one of kPlaqWalks (the
+// autotuner's variants); longer walks carry more links, shorter ones give more blocks.
 template <typename FV>
-__global__ void __launch_bounds__(kBS) plaquette_kernel(Lat L, FV f, int mu0, int mu1, int nu1, uint32_t nbs,
+__global__ void __launch_bounds__(kBS) plaquette_kernel(Lat L, FV f, int mu0, int mu1, int nu1, uint32_t nbs, int lt,
                                                         double* __restrict__ partials) {
     using R = typename FV::real;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -46,7 +47,7 @@ __global__ void __launch_bounds__(kBS) plaquette_kernel(Lat L, FV f, int mu0, in
     double acc = 0.0;
     if (p < L.Vs) {
         const int NT = L.n[0];
-        const int t0 = (int)tc * kPlaqLT, t1 = min(t0 + kPlaqLT, NT);
+        const int t0 = (int)tc * lt, t1 = min(t0 + lt, NT);
         int c[4];
         site_coords(L, p, c);
         // forward spatial neighbours within a slice
@@ -244,16 +245,29 @@ PathArg make_path(const int32_t* path, int npath) {
 // blocks per time chunk (the spatial sites of one slice), times the time chunks
 static uint32_t plaquette_slice_blocks(const Lat& L) { return (L.Vs + kBS - 1) / kBS; }
 
-uint32_t plaquette_blocks(const Lat& L) {
-    return plaquette_slice_blocks(L) * (uint32_t)((L.n[0] + kPlaqLT - 1) / kPlaqLT);
+int plaquette_walk(int variant) {
+    return (variant >= 1 && variant <= kPlaquetteVariants) ? kPlaqWalks[variant - 1] : kPlaqWalks[kPlaquetteDefault - 1];
+}
+
+uint32_t plaquette_blocks(const Lat& L, int variant) {
+    const int lt = plaquette_walk(variant);
+    return plaquette_slice_blocks(L) * (uint32_t)((L.n[0] + lt - 1) / lt);
+}
+
+uint32_t plaquette_max_blocks(const Lat& L) {
+    uint32_t m = 0;
+    for (int v = 1; v <= kPlaquetteVariants; ++v) m = std::max(m, plaquette_blocks(L, v));
+    return m;
 }
 
 hipError_t launch_plaquette(const Lat& L, const void* U, int is_f32, int mu0, int mu1, int nu1, double* partials,
-                            hipStream_t st) {
-    const uint32_t g = plaquette_blocks(L), nbs = plaquette_slice_blocks(L);
+                            hipStream_t st, int variant, uint32_t* npart) {
+    const int lt = plaquette_walk(variant);
+    const uint32_t g = plaquette_blocks(L, variant), nbs = plaquette_slice_blocks(L);
     with_view(L.V, U, is_f32, [&](auto f) {
-        plaquette_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, mu0, mu1, nu1, nbs, partials);
+        plaquette_kernel<decltype(f)><<<g, kBS, 0, st>>>(L, f, mu0, mu1, nu1, nbs, lt, partials);
     });
+    if (npart) *npart = g;
     return hipGetLastError();
 }
 

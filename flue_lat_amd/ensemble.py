@@ -52,6 +52,20 @@ class Ensemble:
         lo, hi = int(self.offsets[c]), int(self.offsets[c + 1])
         return GaugeField(self.dims[c], self.precision, self.device, _data=self.data[lo:hi])
 
+    def autotune(self, reps=3):
+        """Time the persistent-grid variants of the ensemble flow on this ensemble (one flow
+        stage of every field, into scratch; the fields are only read) and remember the
+        fastest for flow() on ensembles of this shape.  Returns (variant, ms per stage)."""
+        best = ctypes.c_int32()
+        ms = ctypes.c_double()
+        nbytes = _lib.lib().flue_ensemble_work_bytes(self.n, self._cdims, self.prec)
+        work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().flue_ensemble_autotune(self.n, self._cdims, _ptr(self.data), self.prec, int(reps),
+                                                    ctypes.byref(best), ctypes.byref(ms), _ptr(work), nbytes,
+                                                    _stream()), "flue_ensemble_autotune")
+        return best.value, ms.value
+
     def flow(self, eps, nsteps, measure_every=10, use_graph=True):
         """nsteps RK3 Wilson-flow steps of every configuration, in place.  Returns
         (plaquette sums, B) as arrays of shape (nsteps // measure_every + 1, n): the sum of

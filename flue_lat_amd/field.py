@@ -305,16 +305,22 @@ class GaugeField:
         return out.item() if sync else out
 
     def autotune(self, op="magnetic", reps=5):
-        """Time the kernel variants of `op` on this field and remember the fastest for its
-        (device, precision, extents); returns (variant, ms per call)."""
-        ops = {"magnetic": 0}
+        """Time the variants of `op` on this field and remember the fastest for its (device,
+        precision, extents); later calls of that operation use it.  op: "magnetic" (kernel
+        form x waves per SIMD), "b2_grad" (gather or reverse-mode adjoint), "plaquette" (time
+        slices per thread, also used by wilson_action), "flow" (stencil visit order, used by
+        wilson_flow and wilson_grad).  The field is only read.  Returns (variant, ms per call)."""
+        if op not in _lib.OPS or op == "ensemble_flow":
+            raise ValueError(f"unknown operation {op!r}")
+        code = _lib.OPS[op]
         best = ctypes.c_int32()
         ms = ctypes.c_double()
-        ws = self.workspace()
+        n = _lib.lib().flue_autotune_work_bytes(code, self._cdims, self.prec)
+        ws = torch.empty(max(n, 1), dtype=torch.uint8, device=self.device)
         with torch.cuda.device(self.device):
-            check_status(_lib.lib().flue_autotune(ops[op], self._cdims, _ptr(self.data), self.prec, int(reps),
-                                           ctypes.byref(best), ctypes.byref(ms), _ptr(ws), ws.numel(), _stream()),
-                  "flue_autotune")
+            check_status(_lib.lib().flue_autotune(code, self._cdims, _ptr(self.data), self.prec, int(reps),
+                                                  ctypes.byref(best), ctypes.byref(ms), _ptr(ws), n, _stream()),
+                         "flue_autotune")
         return best.value, ms.value
 
     # ------------------------------------------------------------ gradients and flow
@@ -339,16 +345,17 @@ class GaugeField:
                                               _stream()), "flue_wilson_grad")
         return g
 
-    def magnetic_grad(self):
+    def magnetic_grad(self, variant=0):
         """Lie gradient of the magnetic B (temporal links get 0).  Needs SU(3) links in every
         precision: the library checks them first and FlueError (FLUE_ESU3) is raised for a
-        field that fails the check."""
+        field that fails the check.  variant: 1 = per-link gather, 2 = reverse mode,
+        0 = tuned or reverse mode."""
         g = self._like()
         n = _lib.lib().flue_b2_work_bytes(self._cdims, self.prec)
         work = torch.empty(n, dtype=torch.uint8, device=self.device)
         with torch.cuda.device(self.device):
-            check_status(_lib.lib().flue_b2_grad(self._cdims, _ptr(self.data), self.prec, _ptr(g.data), _ptr(work), n,
-                                          _stream()), "flue_b2_grad")
+            check_status(_lib.lib().flue_b2_grad_ex(self._cdims, _ptr(self.data), self.prec, int(variant),
+                                                    _ptr(g.data), _ptr(work), n, _stream()), "flue_b2_grad_ex")
         return g
 
     def wilson_flow(self, eps, nsteps, use_graph=True):

@@ -309,6 +309,11 @@ int flue_wilson_action(const int32_t dims[4], const void* U, int32_t prec, doubl
 /* G = (beta/6) P_TA(U Sigma) per link, Sigma the staple sum */
 int flue_wilson_grad(const int32_t dims[4], const void* U, int32_t prec, double beta, void* grad, void* stream);
 
+/* flue_b2_grad with an explicit form: 1 = per-link gather, 2 = reverse mode (each leaf
+ * rectangle walked once, its terms summed per link from fixed slots); 0 = tuned or 2 */
+int flue_b2_grad_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, void* grad, void* work,
+                    size_t work_bytes, void* stream);
+
 /* gradient of the magnetic B (flue_magnetic); temporal links get 0.  `work` holds the
  * per-site anti-Hermitian parts of the three spatial-plane leaf sums
  * (flue_b2_work_bytes; NULL = library buffer).  Precondition in every precision: SU(3)
@@ -333,11 +338,28 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
                      void* work, size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------ */
-/* autotuner: kernels with several variants (occupancy targets) are timed with hipEvents   */
-/* on the caller's field; the fastest is remembered per (device, op, precision, extents)  */
-/* and used whenever the variant argument is 0.  The table persists through save/load.    */
+/* autotuner (no reference counterpart): the variants of an operation are timed with       */
+/* hipEvents on the caller's field; the fastest is remembered per (device, op, precision,  */
+/* extents) and used by every later call of that operation (and by the *_ex entries when  */
+/* their variant argument is 0).  The table persists through save/load.  Variants of one   */
+/* operation compute the same quantity: the flow orders and the ensemble grids give        */
+/* bit-identical results, the others agree to rounding (their summation orders differ).    */
+/*   FLUE_OP_MAGNETIC       1..6  kernel form x waves per SIMD (flue_magnetic_ex)           */
+/*   FLUE_OP_B2_GRAD        1..2  per-link gather / reverse mode with slots (flue_b2_grad_ex) */
+/*   FLUE_OP_PLAQUETTE      1..6  time slices per thread 2, 3, 4, 6, 8, 12 (flue_plaquette,   */
+/*                                flue_wilson_action)                                        */
+/*   FLUE_OP_FLOW           1..6  visit order of the stencil blocks: storage order, or       */
+/*                                (z, t) bricks 1x2, 2x2, 1x4, 2x4, 4x4 (flue_flow_stage,     */
+/*                                flue_wilson_flow, flue_wilson_grad)                         */
+/*   FLUE_OP_ENSEMBLE_FLOW  1..4  persistent grid of 1, 2, 4, 8 workgroups per CU            */
+/*                                (flue_ensemble_flow; tuned by flue_ensemble_autotune)      */
 /* ------------------------------------------------------------------ */
-enum { FLUE_OP_MAGNETIC = 0 };
+enum { FLUE_OP_MAGNETIC = 0, FLUE_OP_B2_GRAD = 1, FLUE_OP_PLAQUETTE = 2, FLUE_OP_FLOW = 3, FLUE_OP_ENSEMBLE_FLOW = 4 };
+/* device bytes flue_autotune needs as `ws` for op (0 for an invalid op) */
+size_t flue_autotune_work_bytes(int32_t op, const int32_t dims[4], int32_t prec);
+/* times every variant of op (not FLUE_OP_ENSEMBLE_FLOW) on U, reps calls each after a
+ * warm-up; U is only read.  ws: flue_autotune_work_bytes(op, dims, prec) (NULL = library
+ * buffer).  Waits for the stream. */
 int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec, int32_t reps, int32_t* best,
                   double* best_ms, void* ws, size_t ws_bytes, void* stream);
 int flue_tuned_variant(int32_t op, const int32_t dims[4], int32_t prec, int32_t* variant);
@@ -364,6 +386,13 @@ int flue_ensemble_random(int32_t n, const int32_t* dims, const uint64_t* seeds, 
 int flue_ensemble_flow(int32_t n, const int32_t* dims, void* fields, int32_t prec, double eps, int32_t nsteps,
                        int32_t measure_every, double* d_plaq, double* d_B, int32_t use_graph, void* work,
                        size_t work_bytes, void* stream);
+
+/* FLUE_OP_ENSEMBLE_FLOW for this ensemble: times one flow stage of every field per grid
+ * variant (the fields are only read; work as for flue_ensemble_flow) and remembers the
+ * fastest for later flue_ensemble_flow calls on ensembles of the same shape.  Waits for
+ * the stream. */
+int flue_ensemble_autotune(int32_t n, const int32_t* dims, const void* fields, int32_t prec, int32_t reps,
+                           int32_t* best, double* best_ms, void* work, size_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -119,13 +119,24 @@ def test_no_gpu_is_reported_not_faked(L):
 def test_autotune_table_roundtrip(L, tmp_path):
     import ctypes
     p = tmp_path / "tune.txt"
-    p.write_text("# header\n0 0 1 4 5 6 7 3 0.25\n0 0 0 4 5 6 7 9 0.5\n")
+    p.write_text("# header\n0 0 1 4 5 6 7 3 0.25\n0 0 0 4 5 6 7 9 0.5\n"
+                 "0 1 0 4 5 6 7 2 9.0\n0 1 1 4 5 6 7 3 9.0\n0 3 0 4 5 6 7 5 1.0\n0 4 1 4 5 6 7 4 2.0\n"
+                 "0 9 0 4 5 6 7 1 1.0\n")
     assert L.flue_tune_load(str(p).encode()) == 0
     v = ctypes.c_int32(-1)
     d = (ctypes.c_int32 * 4)(4, 5, 6, 7)
     assert L.flue_tuned_variant(0, d, 1, ctypes.byref(v)) == 0 and v.value == 3
-    # out-of-range variants are ignored on load
+    # out-of-range variants (per operation) and unknown operations are ignored on load
     assert L.flue_tuned_variant(0, d, 0, ctypes.byref(v)) == 0 and v.value == 0
+    assert L.flue_tuned_variant(1, d, 0, ctypes.byref(v)) == 0 and v.value == 2
+    assert L.flue_tuned_variant(1, d, 1, ctypes.byref(v)) == 0 and v.value == 0
+    assert L.flue_tuned_variant(3, d, 0, ctypes.byref(v)) == 0 and v.value == 5
+    assert L.flue_tuned_variant(4, d, 1, ctypes.byref(v)) == 0 and v.value == 4
+    assert L.flue_tuned_variant(9, d, 0, ctypes.byref(v)) == 0 and v.value == 0
+    # work sizes of the tunable operations (no device needed)
+    for op in range(4):
+        assert L.flue_autotune_work_bytes(op, d, 0) > 0
+    assert L.flue_autotune_work_bytes(4, d, 0) == 0 and L.flue_autotune_work_bytes(7, d, 0) == 0
     q = tmp_path / "out.txt"
     assert L.flue_tune_save(str(q).encode()) == 0
     assert "0 0 1 4 5 6 7 3" in q.read_text()
