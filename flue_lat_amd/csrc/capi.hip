@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -105,6 +106,20 @@ int b2_grad_run(const Lat& L, const void* U, int32_t prec, int variant, void* wo
     else
         FLUE_TRY(launch_b2_grad_rev(L, U, f32, work, (char*)work + b2_ah_bytes(L, prec), grad, st));
     return FLUE_OK;
+}
+
+// flow work buffer: the second link buffer, then the FP64 accumulator, then the plaquette parts
+size_t flow_acc_offset(const Lat& L, int32_t prec) {
+    return ((size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8) + 255) & ~(size_t)255;
+}
+
+// one RK3 stage (Luescher: acc <- ca acc + cb Z, (ca, cb) = (0, 1/4), (-17/9, 8/9), (-1, 3/4))
+int flow_stage_run(const Lat& L, const void* Win, int32_t prec, void* Wout, double* acc, double eps, int stage,
+                   int variant, void* T, hipStream_t st) {
+    static const double ca[3] = {0.0, -17.0 / 9.0, -1.0};
+    static const double cb[3] = {0.25, 8.0 / 9.0, 0.75};
+    return hip_status(launch_flow_stage(L, Win, prec == FLUE_F32, Wout, acc, eps, ca[stage], cb[stage], stage == 2, st,
+                                        variant, T));
 }
 
 // largest accepted max|U U^+ - 1| and |det U - 1|: a few hundred FP32 roundings of a unit
@@ -704,9 +719,10 @@ size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec) {
 
 size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec) {
     if (check_dims(dims) || check_prec(prec)) return 0;
-    const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
-    // a second link buffer and the FP64 RK3 accumulator (8 su(3) coordinates per link)
-    return (size_t)(V * kPlanes * (prec == FLUE_F32 ? 4 : 8) + V * 32 * 8);
+    const Lat L = make_lat(dims);
+    // a second link buffer, the FP64 RK3 accumulator (8 su(3) coordinates per link) and the
+    // plaquette parts of the plaquette-first stencil
+    return flow_acc_offset(L, prec) + (size_t)L.V * 32 * 8 + plaq_ta_bytes(L, prec == FLUE_F32);
 }
 
 int flue_wilson_action(const int32_t dims[4], const void* U, int32_t prec, double beta, double* d_S, void* ws,
@@ -727,12 +743,31 @@ int flue_wilson_action(const int32_t dims[4], const void* U, int32_t prec, doubl
 }
 
 int flue_wilson_grad(const int32_t dims[4], const void* U, int32_t prec, double beta, void* grad, void* stream) {
+    return flue_wilson_grad_ex(dims, U, prec, beta, 0, grad, nullptr, 0, stream);
+}
+
+size_t flue_pf_work_bytes(const int32_t dims[4], int32_t prec) {
+    if (check_dims(dims) || check_prec(prec)) return 0;
+    return plaq_ta_bytes(make_lat(dims), prec == FLUE_F32);
+}
+
+int flue_wilson_grad_ex(const int32_t dims[4], const void* U, int32_t prec, double beta, int32_t variant, void* grad,
+                        void* ws, size_t ws_bytes, void* stream) {
     int rc = check_dims(dims);
     if (rc) return rc;
     if (!U || !grad) return FLUE_ENULL;
-    if (check_prec(prec) || !std::isfinite(beta)) return FLUE_EINVAL;
-    return hip_status(launch_wilson_grad(make_lat(dims), U, prec == FLUE_F32, beta, grad, S(stream),
-                                         variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault)));
+    if (check_prec(prec) || !std::isfinite(beta) || variant < 0 || variant > kFlowVariants) return FLUE_EINVAL;
+    const Lat L = make_lat(dims);
+    if (variant == 0) variant = variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault);
+    // the plaquette-first form needs unitary links (FP32 and declared-SU(3) fields) and a
+    // workspace; otherwise the staple form in storage order
+    void* T = nullptr;
+    if (variant == kFlowPF || variant == kFlowPFWalk) {
+        if (prec == FLUE_F64 || !ws) variant = kFlowDefault;
+        else if (ws_bytes < plaq_ta_bytes(L, prec == FLUE_F32)) return FLUE_EWS;
+        else T = ws;
+    }
+    return hip_status(launch_wilson_grad(L, U, prec == FLUE_F32, beta, grad, S(stream), variant, T));
 }
 
 int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad, void* work, size_t work_bytes,
@@ -772,10 +807,10 @@ int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* 
     if (Win == Wout) return FLUE_EINVAL;
     if (check_prec(prec) || stage < 0 || stage > 2 || !std::isfinite(eps)) return FLUE_EINVAL;
     // Luescher RK3: acc <- ca acc + cb Z, (ca, cb) = (0, 1/4), (-17/9, 8/9), (-1, 3/4)
-    static const double ca[3] = {0.0, -17.0 / 9.0, -1.0};
-    static const double cb[3] = {0.25, 8.0 / 9.0, 0.75};
-    return hip_status(launch_flow_stage(make_lat(dims), Win, prec == FLUE_F32, Wout, acc, eps, ca[stage], cb[stage],
-                                        stage == 2, S(stream), variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault)));
+    // a single stage has no workspace for the plaquette-first form: staple sums
+    int variant = variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault);
+    if (variant == kFlowPF || variant == kFlowPFWalk) variant = kFlowDefault;
+    return flow_stage_run(make_lat(dims), Win, prec, Wout, acc, eps, stage, variant, nullptr, S(stream));
 }
 
 int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, int32_t nsteps, int32_t use_graph,
@@ -788,13 +823,15 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
     if ((rc = get_ws(work, work_bytes, flue_flow_work_bytes(dims, prec), &w))) return rc;
     const Lat L = make_lat(dims);
     void* W2 = w;
-    double* acc = (double*)((char*)w + (size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8));
+    double* acc = (double*)((char*)w + flow_acc_offset(L, prec));
+    void* T = (char*)acc + (size_t)L.V * 32 * 8;
+    const int variant = variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault);
     hipStream_t st = S(stream);
     // three stages per step alternate between U and W2, so two steps return the field to U
     auto two_steps = [&]() -> int {
         void* bufs[2] = {U, W2};
         for (int k = 0; k < 6; ++k) {
-            int r = flue_flow_stage(dims, bufs[k & 1], prec, bufs[(k + 1) & 1], acc, eps, k % 3, st);
+            int r = flow_stage_run(L, bufs[k & 1], prec, bufs[(k + 1) & 1], acc, eps, k % 3, variant, T, st);
             if (r) return r;
         }
         return FLUE_OK;
@@ -834,7 +871,7 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
         for (int k = 0; k < 3; ++k) {
             void* in = (k % 2 == 0) ? U : W2;
             void* out = (k % 2 == 0) ? W2 : U;
-            if ((rc = flue_flow_stage(dims, in, prec, out, acc, eps, k, st))) return rc;
+            if ((rc = flow_stage_run(L, in, prec, out, acc, eps, k, variant, T, st))) return rc;
         }
         FLUE_TRY(hipMemcpyAsync(U, W2, (size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8), hipMemcpyDeviceToDevice,
                                 st));
@@ -868,6 +905,8 @@ int time_variants(int nvar, int reps, hipStream_t st, Fn&& launch, int* best, do
         if (rc == FLUE_OK && e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
         if (rc == FLUE_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (rc == FLUE_OK && e == hipSuccess && std::getenv("FLUE_TUNE_LOG"))
+            std::fprintf(stderr, "flue_autotune: variant %d %.4f ms per call\n", v, (double)ms / reps);
         if (rc == FLUE_OK && e == hipSuccess && (bv == 0 || ms < bms)) {
             bv = v;
             bms = ms;
@@ -899,7 +938,8 @@ size_t flue_autotune_work_bytes(int32_t op, const int32_t dims[4], int32_t prec)
     case FLUE_OP_MAGNETIC: return partial_bytes(magnetic_blocks(L, prec == FLUE_F32));
     case FLUE_OP_PLAQUETTE: return partial_bytes(plaquette_max_blocks(L));
     case FLUE_OP_B2_GRAD: return flue_b2_work_bytes(dims, prec) + fb;  // work, then a gradient
-    case FLUE_OP_FLOW: return fb + (size_t)L.V * 32 * sizeof(double);  // output links, accumulator
+    case FLUE_OP_FLOW:  // output links, accumulator, plaquette parts
+        return flow_acc_offset(L, prec) + (size_t)L.V * 32 * sizeof(double) + plaq_ta_bytes(L, prec == FLUE_F32);
     default: return 0;
     }
 }
@@ -921,17 +961,17 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
     void* w;
     if ((rc = get_ws(ws, ws_bytes, flue_autotune_work_bytes(op, dims, prec), &w))) return rc;
     hipStream_t st = S(stream);
-    const size_t fb = flue_field_bytes(dims, prec);
     if (op == FLUE_OP_B2_GRAD && (rc = run_su3_check(L, U, prec, 0.0, nullptr, w, st))) return rc;
-    if (op == FLUE_OP_FLOW) FLUE_TRY(hipMemsetAsync((char*)w + fb, 0, (size_t)L.V * 32 * sizeof(double), st));
+    double* facc = (double*)((char*)w + flow_acc_offset(L, prec));
     auto launch = [&](int v) -> int {
         switch (op) {
         case FLUE_OP_MAGNETIC: return hip_status(launch_magnetic(L, U, is_f32, su3, (double*)w, st, v, nullptr));
         case FLUE_OP_PLAQUETTE: return hip_status(launch_plaquette(L, U, is_f32, 0, 3, 3, (double*)w, st, v, nullptr));
         case FLUE_OP_B2_GRAD: return b2_grad_run(L, U, prec, v, w, (char*)w + flue_b2_work_bytes(dims, prec), st);
-        default:  // FLUE_OP_FLOW: the middle RK3 stage (it reads the accumulator), into scratch links
-            return hip_status(launch_flow_stage(L, U, is_f32, w, (double*)((char*)w + fb), 0.01, -17.0 / 9.0,
-                                                8.0 / 9.0, 0, st, v));
+        default:  // FLUE_OP_FLOW: the first RK3 stage into scratch links.  It only writes the
+                  // accumulator: a later stage would scale it by -17/9 or -1 on every timed
+                  // call, and the growing exponent would slow each variant more than the last.
+            return flow_stage_run(L, U, prec, w, facc, 0.01, 0, v, (char*)facc + (size_t)L.V * 32 * 8, st);
         }
     };
     int bv = 0;
@@ -1183,15 +1223,15 @@ int flue_ensemble_autotune(int32_t n, const int32_t* dims, const void* fields, i
     const EnsWork ew = ens_work(w, n, p, sz);
     hipStream_t st = S(stream);
     FLUE_TRY(hipMemcpyAsync(ew.tab, p.tab.data(), sizeof(EnsEntry) * n, hipMemcpyHostToDevice, st));
-    FLUE_TRY(hipMemsetAsync(ew.acc, 0, (size_t)p.acc_doubles * 8, st));
     FLUE_TRY(hipStreamSynchronize(st));
-    // the middle RK3 stage of every field into the second buffer: the fields are not changed
+    // the first RK3 stage of every field into the second buffer (the fields are not changed;
+    // it only writes the accumulator, so repeated calls do not drift, as flue_autotune)
     auto launch = [&](int v) -> int {
         uint32_t grid = 0;
         int r = ens_grid(p, v, &grid);
         if (r) return r;
         return hip_status(launch_ens_flow_stage(ew.tab, n, p.total_blocks, p.maxV, fields, is_f32, ew.W2, ew.acc, 0.01,
-                                                -17.0 / 9.0, 8.0 / 9.0, 0, grid, st));
+                                                0.0, 0.25, 0, grid, st));
     };
     int bv = 0;
     double bms = 0.0;

@@ -251,6 +251,103 @@ __device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real
     FLUE_SCHED_FENCE();
 }
 
+// Per-thread parking slots in LDS for chain matrices that wait across a row of the
+// quadrant: two 3x3 complex matrices per thread, laid out [slot][18 reals][thread] so a
+// wave's stores and loads are consecutive.  A thread only touches its own column, so no
+// barrier is needed.
+template <typename R, int BS>
+struct Park {
+    R* col;  // &mem[threadIdx.x] of a __shared__ R mem[2 * 18 * BS]
+    __device__ __forceinline__ void put(int slot, const Mat<R>& m) const {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            col[(slot * 18 + e) * BS] = m.r[e];
+            col[(slot * 18 + 9 + e) * BS] = m.i[e];
+        }
+    }
+    __device__ __forceinline__ Mat<R> get(int slot) const {
+        Mat<R> m;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            m.r[e] = col[(slot * 18 + e) * BS];
+            m.i[e] = col[(slot * 18 + 9 + e) * BS];
+        }
+        return m;
+    }
+};
+
+// The seven rectangles of quadrant() with the chains that wait for a later row parked in
+// LDS: CB(1, .) from its first leaf on and CB(2, .) between rows 1 and 2; CB(3, .) is built
+// again in row 3 (three products more than quadrant() without line hand-over).  At most
+// about five matrices are live at once, so the FP64 form fits two waves per SIMD.
+template <int S1, int S2, bool C2, typename FV, typename PK>
+__device__ __forceinline__ void quadrant_parked(const Quad<FV>& q, AH<typename FV::real>& h, const PK& pk) {
+    using R = typename FV::real;
+    const R w11 = R(kK1), w22 = R(kK2), w33 = R(kK5), w12 = R(0.5 * kK3), w13 = R(0.5 * kK4);
+    // row a = 1
+    Mat<R> CA = ext<S2, 2, 1, 0, C2>(q, first_link<S1, 1, 0, 0>(q));  // CA(1,1)
+    Mat<R> Ld = first_link<S2, 2, 0, 0>(q);                            // Line(x,d2,1)
+    {
+        const Mat<R> CB = ext<S1, 1, 0, 1, C2>(q, Ld);                 // CB(1,1)
+        acc_leaf<C2>(h, w11, CA, CB);
+        pk.put(0, CB);
+    }
+    FLUE_SCHED_FENCE();
+    CA = ext<S2, 2, 1, 1, C2>(q, CA);                                  // CA(1,2)
+    Ld = ext<S2, 2, 0, 1, C2>(q, Ld);                                  // Line(x,d2,2)
+    {
+        const Mat<R> CB = ext<S1, 1, 0, 2, C2>(q, Ld);                 // CB(2,1)
+        acc_leaf<C2>(h, w12, CA, CB);
+        pk.put(1, CB);
+    }
+    FLUE_SCHED_FENCE();
+    CA = ext<S2, 2, 1, 2, C2>(q, CA);                                  // CA(1,3)
+    Ld = ext<S2, 2, 0, 2, C2>(q, Ld);                                  // Line(x,d2,3)
+    acc_leaf<C2>(h, w13, CA, ext<S1, 1, 0, 3, C2>(q, Ld));             // CB(3,1)
+    FLUE_SCHED_FENCE();
+    // row a = 2
+    Mat<R> L = ext<S1, 1, 1, 0, C2>(q, first_link<S1, 1, 0, 0>(q));   // Line(x,d1,2)
+    CA = ext<S2, 2, 2, 0, C2>(q, L);                                   // CA(2,1)
+    {
+        const Mat<R> CB = ext<S1, 1, 1, 1, C2>(q, pk.get(0));          // CB(1,2)
+        acc_leaf<C2>(h, w12, CA, CB);
+        pk.put(0, CB);
+    }
+    FLUE_SCHED_FENCE();
+    CA = ext<S2, 2, 2, 1, C2>(q, CA);                                  // CA(2,2)
+    acc_leaf<C2>(h, w22, CA, ext<S1, 1, 1, 2, C2>(q, pk.get(1)));      // CB(2,2)
+    FLUE_SCHED_FENCE();
+    // row a = 3
+    L = ext<S1, 1, 2, 0, C2>(q, L);                                    // Line(x,d1,3)
+    CA = ext<S2, 2, 3, 0, C2>(q, L);                                   // CA(3,1)
+    acc_leaf<C2>(h, w13, CA, ext<S1, 1, 2, 1, C2>(q, pk.get(0)));      // CB(1,3)
+    FLUE_SCHED_FENCE();
+    CA = ext<S2, 2, 3, 1, C2>(q, CA);                                  // CA(3,2)
+    CA = ext<S2, 2, 3, 2, C2>(q, CA);                                  // CA(3,3)
+    FLUE_SCHED_FENCE();
+    {
+        Mat<R> CB = ext<S2, 2, 0, 1, C2>(q, first_link<S2, 2, 0, 0>(q));  // Line(x,d2,2)
+        CB = ext<S2, 2, 0, 2, C2>(q, CB);                                  // Line(x,d2,3)
+        CB = ext<S1, 1, 0, 3, C2>(q, CB);                                  // CB(3,1)
+        CB = ext<S1, 1, 1, 3, C2>(q, CB);                                  // CB(3,2)
+        CB = ext<S1, 1, 2, 3, C2>(q, CB);                                  // CB(3,3)
+        acc_leaf<C2>(h, w33, CA, CB);
+    }
+    FLUE_SCHED_FENCE();
+}
+
+// plane_ah with every quadrant in its parked form
+template <bool C2, typename FV, typename PK>
+__device__ __forceinline__ AH<typename FV::real> plane_ah_parked(const PlaneGeom<FV>& g, const PK& pk) {
+    AH<typename FV::real> h;
+    ah_zero(h);
+    quadrant_parked<1, 1, C2>(make_quad<1, 1, true>(g), h, pk);
+    quadrant_parked<1, -1, C2>(make_quad<1, -1, false>(g), h, pk);
+    quadrant_parked<-1, -1, C2>(make_quad<-1, -1, true>(g), h, pk);
+    quadrant_parked<-1, 1, C2>(make_quad<-1, 1, false>(g), h, pk);
+    return h;
+}
+
 // the four quadrants in counter-clockwise order: (+mu,+nu), (+nu,-mu), (-mu,-nu), (-nu,+mu).
 // Each quadrant's d2 is the next one's d1; with SHARE the lines along it are computed once.
 // With C2 the link chains carry two rows (SU(3) links only; see mul2).

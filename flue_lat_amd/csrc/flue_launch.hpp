@@ -82,13 +82,19 @@ hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, dou
 // gradients and flow (k_grad.hip, k_b2grad.hip); grad / Wout / Abuf are SoA in the field
 // precision, acc is FP64 (8 su(3) coordinates x 4 directions x V)
 uint32_t grad_blocks(const Lat& L);
-// stencil visit orders (k_grad.hip): 1 = storage order, 2..6 = (z, t) bricks
-constexpr int kFlowVariants = 6;
+// stencil variants (k_grad.hip): 1 = staple sums in storage order, 2..6 = staple sums in
+// (z, t) brick orders, 7 = plaquette-first (the six plaquette parts of every site into T,
+// plaq_ta_bytes, then each link from 3 links and 6 stored parts; unitary links only),
+// 8 = plaquette-first with both passes walking along t
+constexpr int kFlowVariants = 8;
+constexpr int kFlowPF = 7;
+constexpr int kFlowPFWalk = 8;
 constexpr int kFlowDefault = 1;
+size_t plaq_ta_bytes(const Lat& L, int is_f32);
 hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st,
-                              int variant);
+                              int variant, void* T);
 hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, double* acc, double eps,
-                             double ca, double cb, int reunit, hipStream_t st, int variant);
+                             double ca, double cb, int reunit, hipStream_t st, int variant, void* T);
 hipError_t launch_magnetic_ah(const Lat& L, const void* U, int is_f32, void* Abuf, hipStream_t st);
 hipError_t launch_b2_grad(const Lat& L, const void* U, int is_f32, const void* Abuf, void* grad, hipStream_t st);
 // reverse-mode B^2 adjoint: slab by slab of time slices, per-rectangle terms into `slots`

@@ -75,17 +75,144 @@ __global__ void __launch_bounds__(kBS) flow_stage_kernel(Lat L, FV f, BlockOrder
     flow_link(L, f, wout, acc, s, c, mu, eps, ca, cb, reunit);
 }
 
+// plaquette-first stencil (kFlowPF): the plaquette parts of every site, then per link
+template <typename FV>
+__global__ void __launch_bounds__(kBS) plaq_ta_kernel(Lat L, FV f, typename FV::real* __restrict__ T) {
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    if (s >= L.V) return;
+    plaquette_ta_site(L, f, s, T);
+}
+
+template <typename FV>
+__global__ void __launch_bounds__(kBS) wilson_grad_pf_kernel(Lat L, FV f, const typename FV::real* __restrict__ T,
+                                                             double beta, typename FV::real* __restrict__ grad) {
+    using R = typename FV::real;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int mu = (int)(blk & 3u);
+    const uint32_t s = (blk >> 2) * kBS + threadIdx.x;
+    if (s >= L.V) return;
+    int c[4];
+    site_coords(L, s, c);
+    Mat<R> G = ta_force(L, f, T, s, c, mu);
+    const R sc = (R)(beta / 6.0);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        G.r[e] *= sc;
+        G.i[e] *= sc;
+    }
+    store_link(grad, L.V, mu, s, G);
+}
+
+template <typename FV>
+__global__ void __launch_bounds__(kBS) flow_stage_pf_kernel(Lat L, FV f, const typename FV::real* __restrict__ T,
+                                                            typename FV::real* __restrict__ wout,
+                                                            double* __restrict__ acc, double eps, double ca, double cb,
+                                                            int reunit) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int mu = (int)(blk & 3u);
+    const uint32_t s = (blk >> 2) * kBS + threadIdx.x;
+    if (s >= L.V) return;
+    int c[4];
+    site_coords(L, s, c);
+    flow_link_pf(L, f, T, wout, acc, s, c, mu, eps, ca, cb, reunit);
+}
+
+// Plaquette-first with walks along t (kFlowPFWalk).  Pass 1: one thread per spatial site walks
+// kPFWalk slices, keeping the spatial links at x + t in registers as the next slice's own
+// links (13 link loads per site and slice, none from another slice).  Pass 2: one thread per
+// (spatial site, direction) walks the same slices, carrying the previous slice's T_{0 mu} and
+// U_0 for the t term of a spatial link, so no thread reads another slice after its first.
+constexpr int kPFWalk = 6;
+
+template <typename FV>
+__global__ void __launch_bounds__(kBS) plaq_ta_walk_kernel(Lat L, FV f, uint32_t nbs, typename FV::real* __restrict__ T) {
+    using R = typename FV::real;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t tc = blk / nbs, pb = blk - tc * nbs;
+    const uint32_t p = pb * kBS + threadIdx.x;
+    if (p >= L.Vs) return;
+    const int NT = L.n[0];
+    const int t0 = (int)tc * kPFWalk, t1 = min(t0 + kPFWalk, NT);
+    int c[4];
+    site_coords(L, p, c);
+    uint32_t sf[4];  // forward spatial neighbours within a slice
+#pragma unroll
+    for (int d = 1; d < 4; ++d) sf[d] = c[d] == L.n[d] - 1 ? p - (uint32_t)(L.n[d] - 1) * L.st[d] : p + L.st[d];
+    Mat<R> Us[3];
+#pragma unroll
+    for (int b = 1; b < 4; ++b) Us[b - 1] = load_link(f, b, p + L.Vs * (uint32_t)t0);
+    for (int t = t0; t < t1; ++t) {
+        const uint32_t ts = L.Vs * (uint32_t)t, s = p + ts;
+        const uint32_t sn = t == NT - 1 ? p : s + L.Vs;  // x + t
+        const Mat<R> Ut = load_link(f, 0, s);
+        Mat<R> N[3];
+#pragma unroll
+        for (int b = 1; b < 4; ++b) {
+            N[b - 1] = load_link(f, b, sn);
+            store_plaq_ta(T, L.V, plaq_index(0, b), s, mul(Ut, N[b - 1]), mul(Us[b - 1], load_link(f, 0, sf[b] + ts)));
+        }
+#pragma unroll
+        for (int a = 1; a < 3; ++a)
+#pragma unroll
+            for (int b = a + 1; b < 4; ++b)
+                store_plaq_ta(T, L.V, plaq_index(a, b), s, mul(Us[a - 1], load_link(f, b, sf[a] + ts)),
+                              mul(Us[b - 1], load_link(f, a, sf[b] + ts)));
+#pragma unroll
+        for (int b = 0; b < 3; ++b) Us[b] = N[b];
+    }
+}
+
+// pass 2 of kFlowPFWalk for the flow (GRAD = false) or the Wilson gradient (GRAD = true)
+template <bool GRAD, typename FV>
+__global__ void __launch_bounds__(kBS) pf_walk_kernel(Lat L, FV f, const typename FV::real* __restrict__ T,
+                                                      uint32_t nbs, typename FV::real* __restrict__ out,
+                                                      double* __restrict__ acc, double eps, double ca, double cb,
+                                                      int reunit, double beta) {
+    using R = typename FV::real;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int mu = (int)(blk & 3u);
+    const uint32_t rest = blk >> 2, tc = rest / nbs, pb = rest - tc * nbs;
+    const uint32_t p = pb * kBS + threadIdx.x;
+    if (p >= L.Vs) return;
+    const int NT = L.n[0];
+    const int t0 = (int)tc * kPFWalk, t1 = min(t0 + kPFWalk, NT);
+    int c[4];
+    site_coords(L, p, c);
+    Mat<R> Tprev, U0prev;
+    if (mu != 0) {
+        const uint32_t sp = p + L.Vs * (uint32_t)(t0 == 0 ? NT - 1 : t0 - 1);
+        Tprev = load_ta(T, L.V, plaq_index(0, mu), sp);
+        U0prev = load_link(f, 0, sp);
+    }
+    for (int t = t0; t < t1; ++t) {
+        const uint32_t s = p + L.Vs * (uint32_t)t;
+        c[0] = t;
+        Mat<R> Z = ta_force_walk(L, f, T, s, c, mu, Tprev, U0prev);
+        if constexpr (GRAD) {
+            const R sc = (R)(beta / 6.0);
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                Z.r[e] *= sc;
+                Z.i[e] *= sc;
+            }
+            store_link(out, L.V, mu, s, Z);
+        } else {
+            flow_update(L, f, out, acc, s, mu, eps, ca, cb, reunit, Z);
+        }
+    }
+}
+
 }  // namespace
 
-// brick extents (z-planes, t-slices) of the stencil variants; variant 1 is the storage order
-constexpr uint32_t kBrickZ[kFlowVariants] = {0, 1, 2, 1, 2, 4};
-constexpr uint32_t kBrickT[kFlowVariants] = {0, 2, 2, 4, 4, 4};
+// brick extents (z-planes, t-slices) of the staple-form variants; variant 1 is the storage order
+constexpr uint32_t kBrickZ[kFlowPF - 1] = {0, 1, 2, 1, 2, 4};  // variants 1..6
+constexpr uint32_t kBrickT[kFlowPF - 1] = {0, 2, 2, 4, 4, 4};
 
 // the order of a variant, or the storage order where the bricks do not tile the lattice
 // (planes of a whole number of blocks, extents divisible by the brick)
 static BlockOrder block_order(const Lat& L, int variant) {
     BlockOrder bo{0, 0, 1, 1};
-    if (variant < 1 || variant > kFlowVariants) variant = kFlowDefault;
+    if (variant < 1 || variant >= kFlowPF) variant = kFlowDefault;  // the plaquette-first forms order their own blocks
     const uint32_t bz = kBrickZ[variant - 1], bt = kBrickT[variant - 1];
     const uint32_t plane = (uint32_t)L.n[1] * (uint32_t)L.n[2];
     if (bz == 0 || plane % kBS != 0 || L.n[3] % bz != 0 || L.n[0] % bt != 0) return bo;
@@ -95,24 +222,50 @@ static BlockOrder block_order(const Lat& L, int variant) {
 // four blocks (one per direction) per 256 sites
 uint32_t grad_blocks(const Lat& L) { return 4u * ((L.V + kBS - 1) / kBS); }
 
+size_t plaq_ta_bytes(const Lat& L, int is_f32) { return (size_t)48 * L.V * (is_f32 ? 4 : 8); }
+
 hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st,
-                              int variant) {
+                              int variant, void* T) {
     const uint32_t g = grad_blocks(L);
+    if ((variant == kFlowPF || variant == kFlowPFWalk) && !T) return hipErrorInvalidValue;
     const BlockOrder bo = block_order(L, variant);
+    const uint32_t nbs = (L.Vs + kBS - 1) / kBS, ntc = ((uint32_t)L.n[0] + kPFWalk - 1) / kPFWalk;
     with_view(L.V, U, is_f32, [&](auto f) {
         using F = decltype(f);
-        wilson_grad_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, beta, (typename F::real*)grad);
+        using R = typename F::real;
+        if (variant == kFlowPFWalk) {
+            plaq_ta_walk_kernel<F><<<nbs * ntc, kBS, 0, st>>>(L, f, nbs, (R*)T);
+            pf_walk_kernel<true, F><<<4 * nbs * ntc, kBS, 0, st>>>(L, f, (const R*)T, nbs, (R*)grad, nullptr, 0.0, 0.0,
+                                                                  0.0, 0, beta);
+        } else if (variant == kFlowPF) {
+            plaq_ta_kernel<F><<<(L.V + kBS - 1) / kBS, kBS, 0, st>>>(L, f, (R*)T);
+            wilson_grad_pf_kernel<F><<<g, kBS, 0, st>>>(L, f, (const R*)T, beta, (R*)grad);
+        } else {
+            wilson_grad_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, beta, (R*)grad);
+        }
     });
     return hipGetLastError();
 }
 
 hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, double* acc, double eps,
-                             double ca, double cb, int reunit, hipStream_t st, int variant) {
+                             double ca, double cb, int reunit, hipStream_t st, int variant, void* T) {
     const uint32_t g = grad_blocks(L);
+    if ((variant == kFlowPF || variant == kFlowPFWalk) && !T) return hipErrorInvalidValue;
     const BlockOrder bo = block_order(L, variant);
+    const uint32_t nbs = (L.Vs + kBS - 1) / kBS, ntc = ((uint32_t)L.n[0] + kPFWalk - 1) / kPFWalk;
     with_view(L.V, Win, is_f32, [&](auto f) {
         using F = decltype(f);
-        flow_stage_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, (typename F::real*)Wout, acc, eps, ca, cb, reunit);
+        using R = typename F::real;
+        if (variant == kFlowPFWalk) {
+            plaq_ta_walk_kernel<F><<<nbs * ntc, kBS, 0, st>>>(L, f, nbs, (R*)T);
+            pf_walk_kernel<false, F><<<4 * nbs * ntc, kBS, 0, st>>>(L, f, (const R*)T, nbs, (R*)Wout, acc, eps, ca, cb,
+                                                                   reunit, 0.0);
+        } else if (variant == kFlowPF) {
+            plaq_ta_kernel<F><<<(L.V + kBS - 1) / kBS, kBS, 0, st>>>(L, f, (R*)T);
+            flow_stage_pf_kernel<F><<<g, kBS, 0, st>>>(L, f, (const R*)T, (R*)Wout, acc, eps, ca, cb, reunit);
+        } else {
+            flow_stage_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, (R*)Wout, acc, eps, ca, cb, reunit);
+        }
     });
     return hipGetLastError();
 }

@@ -336,13 +336,18 @@ class GaugeField:
                                                 _ptr(ws), ws.numel(), _stream()), "flue_wilson_action")
         return out.item() if sync else out
 
-    def wilson_grad(self, beta):
+    def wilson_grad(self, beta, variant=0):
         """Lie gradient of the Wilson action, G = (beta/6) P_TA(U Sigma), as a GaugeField-shaped
-        su(3) field (same SoA layout and precision)."""
+        su(3) field (same SoA layout and precision).  variant: the stencil (0 = tuned; 1..6
+        staple sums in the autotuner's visit orders; 7 plaquette-first, used for FP32 and
+        "fp64-su3" fields, whose links are SU(3))."""
         g = self._like()
+        n = _lib.lib().flue_pf_work_bytes(self._cdims, self.prec)
+        ws = torch.empty(n, dtype=torch.uint8, device=self.device)
         with torch.cuda.device(self.device):
-            check_status(_lib.lib().flue_wilson_grad(self._cdims, _ptr(self.data), self.prec, float(beta), _ptr(g.data),
-                                              _stream()), "flue_wilson_grad")
+            check_status(_lib.lib().flue_wilson_grad_ex(self._cdims, _ptr(self.data), self.prec, float(beta),
+                                                        int(variant), _ptr(g.data), _ptr(ws), n, _stream()),
+                         "flue_wilson_grad_ex")
         return g
 
     def magnetic_grad(self, variant=0):

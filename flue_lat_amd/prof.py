@@ -5,6 +5,11 @@ plain device copy of known size.  Meant to sit under a profiler:
     rocprofv3 --kernel-trace --stats -d out -- python3 -m flue_lat_amd.prof
     rocprofv3 --pmc FETCH_SIZE -d out -- python3 -m flue_lat_amd.prof --prec fp32
     rocprofv3 --kernel-trace --stats -d out -- python3 -m flue_lat_amd.prof --op all --copy-mib 0
+    rocprofv3 --kernel-trace --stats -d out -- python3 -m flue_lat_amd.prof --op flow --variant 7
+
+--variant N runs variant N of the operation: the magnetic kernel form, the B^2 adjoint form,
+the plaquette walk length, or the flow/gradient stencil (through a tuning table written for
+the field's extents and precisions before the run).
 """
 import argparse
 import sys
@@ -18,13 +23,26 @@ def main(argv=None):
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the calibration copy (0 = none)")
     ops = ["magnetic", "plaquette", "polyakov", "fmunu", "wloops", "propagator", "wilson_grad", "flow", "b2_grad"]
     ap.add_argument("--op", choices=ops + ["all"], default="magnetic")
+    ap.add_argument("--variant", type=int, default=0, help="variant of the operation (0 = tuned or default)")
     a = ap.parse_args(argv)
+
+    import os
+    import tempfile
 
     import torch
 
-    from .field import GaugeField
+    from . import _lib
+    from .field import GaugeField, tune_load
 
     torch.cuda.set_device(0)
+    tunable = {"plaquette": "plaquette", "flow": "flow", "wilson_grad": "flow", "b2_grad": "b2_grad"}
+    if a.variant and a.op in tunable:
+        code = _lib.OPS[tunable[a.op]]
+        with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as t:
+            for prec in (0, 1, 2):
+                t.write(f"0 {code} {prec} {' '.join(str(d) for d in a.dims)} {a.variant} 1.0\n")
+        tune_load(t.name)
+        os.unlink(t.name)
     f64 = GaugeField.random(a.dims, 1003, 0.3, "fp64")
     fields = []
     if a.prec in ("fp32", "both"):
@@ -34,7 +52,7 @@ def main(argv=None):
 
     def run(op, g):
         if op == "magnetic":
-            return g.magnetic()
+            return g.magnetic(variant=a.variant)
         if op == "plaquette":
             return g.plaquette()[0]
         if op == "polyakov":

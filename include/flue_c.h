@@ -306,8 +306,22 @@ size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec);
 int flue_wilson_action(const int32_t dims[4], const void* U, int32_t prec, double beta, double* d_S, void* ws,
                        size_t ws_bytes, void* stream);
 
-/* G = (beta/6) P_TA(U Sigma) per link, Sigma the staple sum */
+/* G = (beta/6) P_TA(U Sigma) per link, Sigma the staple sum (the tuned FLUE_OP_FLOW visit
+ * order of the staple form) */
 int flue_wilson_grad(const int32_t dims[4], const void* U, int32_t prec, double beta, void* grad, void* stream);
+
+/* bytes of the plaquette parts the plaquette-first stencil keeps (48 reals per site) */
+size_t flue_pf_work_bytes(const int32_t dims[4], int32_t prec);
+
+/* flue_wilson_grad with an explicit FLUE_OP_FLOW variant (0 = tuned): 1..6 staple sums in
+ * the visit orders of the autotuner table, 7 = plaquette-first: the six plaquettes of every
+ * site are formed once and their traceless anti-Hermitian parts stored in ws
+ * (flue_pf_work_bytes), then each link takes its up plaquettes at x and its down plaquettes
+ * at x - nu conjugated by U_nu(x - nu).  That identity holds for unitary links, so variant 7
+ * runs for FLUE_F32 and FLUE_F64_SU3 fields given a workspace; FLUE_F64 fields and ws == NULL
+ * take variant 1. */
+int flue_wilson_grad_ex(const int32_t dims[4], const void* U, int32_t prec, double beta, int32_t variant, void* grad,
+                        void* ws, size_t ws_bytes, void* stream);
 
 /* flue_b2_grad with an explicit form: 1 = per-link gather, 2 = reverse mode (each leaf
  * rectangle walked once, its terms summed per link from fixed slots); 0 = tuned or 2 */
@@ -333,7 +347,10 @@ int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* 
 
 /* nsteps RK3 steps in place on U.  use_graph != 0 captures two steps in a hipGraph and
  * replays it (needs a non-default stream); `work` is flue_flow_work_bytes (NULL = library
- * buffer, which must then exist before any capture). */
+ * buffer, which must then exist before any capture).  The flow moves SU(3) links along the
+ * group (exponential map, reunitarisation after every step), so it takes SU(3) links in
+ * every precision: the tuned FLUE_OP_FLOW variant, including the plaquette-first variant 7,
+ * is used.  flue_flow_stage (no workspace) uses the staple forms only. */
 int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, int32_t nsteps, int32_t use_graph,
                      void* work, size_t work_bytes, void* stream);
 
@@ -348,9 +365,10 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
 /*   FLUE_OP_B2_GRAD        1..2  per-link gather / reverse mode with slots (flue_b2_grad_ex) */
 /*   FLUE_OP_PLAQUETTE      1..6  time slices per thread 2, 3, 4, 6, 8, 12 (flue_plaquette,   */
 /*                                flue_wilson_action)                                        */
-/*   FLUE_OP_FLOW           1..6  visit order of the stencil blocks: storage order, or       */
-/*                                (z, t) bricks 1x2, 2x2, 1x4, 2x4, 4x4 (flue_flow_stage,     */
-/*                                flue_wilson_flow, flue_wilson_grad)                         */
+/*   FLUE_OP_FLOW           1..7  stencil of the flow and the Wilson gradient: staple sums   */
+/*                                in storage order or (z, t) bricks 1x2, 2x2, 1x4, 2x4, 4x4   */
+/*                                (1..6, bit-identical), or plaquette-first (7, agrees to     */
+/*                                rounding; unitary links, see flue_wilson_grad_ex)           */
 /*   FLUE_OP_ENSEMBLE_FLOW  1..4  persistent grid of 1, 2, 4, 8 workgroups per CU            */
 /*                                (flue_ensemble_flow; tuned by flue_ensemble_autotune)      */
 /* ------------------------------------------------------------------ */
@@ -359,7 +377,8 @@ enum { FLUE_OP_MAGNETIC = 0, FLUE_OP_B2_GRAD = 1, FLUE_OP_PLAQUETTE = 2, FLUE_OP
 size_t flue_autotune_work_bytes(int32_t op, const int32_t dims[4], int32_t prec);
 /* times every variant of op (not FLUE_OP_ENSEMBLE_FLOW) on U, reps calls each after a
  * warm-up; U is only read.  ws: flue_autotune_work_bytes(op, dims, prec) (NULL = library
- * buffer).  Waits for the stream. */
+ * buffer).  Waits for the stream.  With the environment variable FLUE_TUNE_LOG set, every
+ * variant's time is printed to stderr. */
 int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec, int32_t reps, int32_t* best,
                   double* best_ms, void* ws, size_t ws_bytes, void* stream);
 int flue_tuned_variant(int32_t op, const int32_t dims[4], int32_t prec, int32_t* variant);

@@ -84,6 +84,33 @@ def test_plaquette_walk_lengths(FL, tmp_path):
     assert abs(g.plaquette()[0] - want[(1, 4, 4)]) < 1e-12 * abs(want[(1, 4, 4)])
 
 
+@pytest.mark.parametrize("variant", [7, 8])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+@pytest.mark.parametrize("dims", [(6, 5, 4, 7), (13, 4, 3, 5), (1, 3, 4, 2), (2, 5, 3, 3)])
+def test_plaquette_first_stencil(FL, tmp_path, prec, tol, variant, dims):
+    # variants 7 and 8 form each plaquette once per site (8 walking along t, with walks of 6
+    # slices: 13 leaves a short last walk, 1 and 2 wrap within a walk); they agree with the
+    # staple sums to rounding
+    U = O.random_field(dims, 6060 + sum(dims), 0.3)
+    code = 0 if prec == "fp64" else 1
+    g = FL.GaugeField.from_complex(U, prec)
+    Go = O.wilson_grad(U, 6.0)
+    scale = np.abs(Go).max()
+    if prec == "fp64":
+        # plain FP64 assumes nothing about the links: the gradient keeps the staple form
+        assert torch.equal(g.wilson_grad(6.0, variant=variant).data, g.wilson_grad(6.0, variant=1).data)
+        gs = FL.GaugeField(dims, "fp64-su3", _data=g.data)
+        assert np.abs(grad_np(gs.wilson_grad(6.0, variant=variant)) - Go).max() < tol * scale
+    else:
+        assert np.abs(grad_np(g.wilson_grad(6.0, variant=variant)) - Go).max() < tol * scale
+    force(FL, tmp_path, "flow", dims, code, variant)
+    W = g.clone()
+    W.wilson_flow(0.02, 3, use_graph=True)
+    Wo = O.wilson_flow(U, 0.02, 3)
+    assert np.abs(W.to_complex().cpu().numpy() - Wo).max() < tol
+    force(FL, tmp_path, "flow", dims, code, 1)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_flow_visit_orders_bit_identical(FL, tmp_path, prec):
     # (x, y) planes of 256 sites, NZ and NT divisible by every brick: all six orders apply
@@ -105,7 +132,7 @@ def test_flow_visit_orders_bit_identical(FL, tmp_path, prec):
             assert torch.equal(G, want_grad), v
             assert torch.equal(W.data, want_flow), v
     b, ms = base.autotune("flow", reps=2)
-    assert 1 <= b <= 6 and ms > 0
+    assert 1 <= b <= 8 and ms > 0
 
 
 def test_magnetic_and_b2_tuning_keep_results(FL):
