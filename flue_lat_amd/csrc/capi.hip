@@ -1068,22 +1068,25 @@ void ens_key(const EnsPlan& p, int32_t n, int32_t key[4]) {
     key[3] = 0;
 }
 
-// persistent grid of the ensemble kernels: 1, 2, 4 or 8 workgroups per CU (variant 1..4)
+// persistent grid of the ensemble kernels: 1, 2, 4 or 8 workgroups per CU (variant 1..4, and
+// 5..8 with the plaquette-first stencil)
 int ens_grid(const EnsPlan& p, int variant, uint32_t* grid) {
     int dev = 0, cus = 0;
     FLUE_TRY(hipGetDevice(&dev));
     FLUE_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const uint32_t per_cu = 1u << (variant - 1);
+    const uint32_t per_cu = 1u << ((variant - 1) % 4);
     *grid = std::max<uint32_t>(1, std::min<uint32_t>(p.total_blocks, (uint32_t)std::max(cus, 1) * per_cu));
     return FLUE_OK;
 }
 
-// the ensemble work buffer: table, second link buffer, FP64 accumulators, partial sums
+// the ensemble work buffer: table, second link buffer, FP64 accumulators, partial sums, and
+// the plaquette parts of the plaquette-first stencil
 struct EnsWork {
     EnsEntry* tab;
     void* W2;
     double* acc;
     double* partials;
+    void* T;
 };
 
 EnsWork ens_work(void* w, int32_t n, const EnsPlan& p, size_t sz) {
@@ -1096,6 +1099,8 @@ EnsWork ens_work(void* w, int32_t n, const EnsPlan& p, size_t sz) {
     r.acc = (double*)base;
     base += align256((size_t)p.acc_doubles * 8);
     r.partials = (double*)base;
+    base += align256((size_t)p.total_blocks * 16);
+    r.T = base;
     return r;
 }
 
@@ -1110,9 +1115,10 @@ size_t flue_ensemble_field_bytes(int32_t n, const int32_t* dims, int32_t prec) {
 size_t flue_ensemble_work_bytes(int32_t n, const int32_t* dims, int32_t prec) {
     EnsPlan p;
     if (ens_plan(n, dims, p) || check_prec(prec)) return 0;
-    // table, second link buffer, FP64 accumulators, per-block partial sums
-    return align256(sizeof(EnsEntry) * n) + align256((size_t)p.field_reals * (prec == FLUE_F32 ? 4 : 8)) +
-           align256((size_t)p.acc_doubles * 8) + align256((size_t)p.total_blocks * 16);
+    // table, second link buffer, FP64 accumulators, per-block partial sums, plaquette parts
+    const size_t sz = prec == FLUE_F32 ? 4 : 8;
+    return align256(sizeof(EnsEntry) * n) + align256((size_t)p.field_reals * sz) + align256((size_t)p.acc_doubles * 8) +
+           align256((size_t)p.total_blocks * 16) + align256((size_t)p.field_reals / 72 * 48 * sz);
 }
 
 int flue_ensemble_random(int32_t n, const int32_t* dims, const uint64_t* seeds, double eps, void* fields, int32_t prec,
@@ -1153,8 +1159,10 @@ int flue_ensemble_flow(int32_t n, const int32_t* dims, void* fields, int32_t pre
 
     int32_t key[4];
     ens_key(p, n, key);
+    const int variant = variant_or(FLUE_OP_ENSEMBLE_FLOW, key, prec, kEnsembleDefault);
     uint32_t grid = 0;
-    if ((rc = ens_grid(p, variant_or(FLUE_OP_ENSEMBLE_FLOW, key, prec, kEnsembleDefault), &grid))) return rc;
+    if ((rc = ens_grid(p, variant, &grid))) return rc;
+    void* T = variant > 4 ? ew.T : nullptr;  // the plaquette-first variants
 
     const int nmeas = measure_every > 0 ? nsteps / measure_every + 1 : 0;
     static const double ca[3] = {0.0, -17.0 / 9.0, -1.0};
@@ -1175,7 +1183,7 @@ int flue_ensemble_flow(int32_t n, const int32_t* dims, void* fields, int32_t pre
             for (int k = 0; k < 3 && e == hipSuccess; ++k) {
                 const int q = 3 * j + k;
                 e = launch_ens_flow_stage(d_tab, n, p.total_blocks, p.maxV, bufs[q & 1], is_f32, bufs[(q + 1) & 1], acc,
-                                          eps, ca[k], cb[k], k == 2, grid, st);
+                                          eps, ca[k], cb[k], k == 2, grid, T, st);
             }
             if (e == hipSuccess && nmeas > 0 && (j + 1) % measure_every == 0)
                 e = measure(bufs[(j + 1) & 1], (j + 1) / measure_every);
@@ -1231,7 +1239,7 @@ int flue_ensemble_autotune(int32_t n, const int32_t* dims, const void* fields, i
         int r = ens_grid(p, v, &grid);
         if (r) return r;
         return hip_status(launch_ens_flow_stage(ew.tab, n, p.total_blocks, p.maxV, fields, is_f32, ew.W2, ew.acc, 0.01,
-                                                0.0, 0.25, 0, grid, st));
+                                                0.0, 0.25, 0, grid, v > 4 ? ew.T : nullptr, st));
     };
     int bv = 0;
     double bms = 0.0;

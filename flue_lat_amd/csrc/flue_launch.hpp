@@ -121,16 +121,19 @@ struct EnsEntry {
     uint32_t blk0;
     uint32_t nblk;
 };
+// T != nullptr: the plaquette-first stencil with the plaquette parts of every field in T
+// (48 reals per site, field c's at 2/3 of its field offset); T == nullptr: staple sums
 hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* win,
                                  int is_f32, void* wout, double* acc, double eps, double ca, double cb, int reunit,
-                                 uint32_t grid, hipStream_t st);
+                                 uint32_t grid, void* T, hipStream_t st);
 hipError_t launch_ens_sum(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* fld, int is_f32,
                           int obs, double* partials, uint32_t grid, hipStream_t st);
 hipError_t launch_ens_finalize(const EnsEntry* tab, int n, const double* partials, double* out, int stride,
                                int per_volume, hipStream_t st);
 hipError_t launch_copy(size_t n_reals, int is_f32, const void* src, void* dst, hipStream_t st);
-// persistent grid of the ensemble kernels: 1, 2, 4 or 8 workgroups per CU
-constexpr int kEnsembleVariants = 4;
+// ensemble flow variants: persistent grid of 1, 2, 4 or 8 workgroups per CU with the staple
+// stencil (1..4) or the plaquette-first stencil (5..8)
+constexpr int kEnsembleVariants = 8;
 constexpr int kEnsembleDefault = 3;
 
 }  // namespace flue

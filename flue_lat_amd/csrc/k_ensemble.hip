@@ -75,6 +75,45 @@ __global__ void __launch_bounds__(kBS) ens_flow_kernel(const EnsEntry* __restric
     }
 }
 
+// Plaquette-first stencil over the ensemble: the six plaquette parts of every site of every
+// field (48 reals per site, config c's block at 2/3 of its field offset), then the links.
+template <typename R, bool BUF>
+__global__ void __launch_bounds__(kBS) ens_plaq_ta_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
+                                                          uint32_t chunk, const R* __restrict__ win, R* __restrict__ T) {
+    uint32_t lo, hi;
+    my_range(total, chunk, lo, hi);
+    for (uint32_t lb = lo; lb < hi; ++lb) {
+        const EnsEntry& e = entry(tab, find_cfg(tab, n, lb));
+        const uint32_t s = (lb - e.blk0) * kBS + threadIdx.x;
+        if (s < e.L.V) {
+            const FieldView<R, BUF> f{win + e.off, e.L.V};
+            plaquette_ta_site(e.L, f, s, T + e.off / 72 * 48);
+        }
+    }
+}
+
+template <typename R, bool BUF>
+__global__ void __launch_bounds__(kBS) ens_flow_pf_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
+                                                          uint32_t chunk, const R* __restrict__ win,
+                                                          const R* __restrict__ T, R* __restrict__ wout,
+                                                          double* __restrict__ acc, double eps, double ca, double cb,
+                                                          int reunit) {
+    uint32_t lo, hi;
+    my_range(4u * total, chunk, lo, hi);
+    for (uint32_t lu = lo; lu < hi; ++lu) {
+        const uint32_t lb = lu >> 2;
+        const int mu = (int)(lu & 3u);
+        const EnsEntry& e = entry(tab, find_cfg(tab, n, lb));
+        const uint32_t s = (lb - e.blk0) * kBS + threadIdx.x;
+        if (s < e.L.V) {
+            const FieldView<R, BUF> f{win + e.off, e.L.V};
+            int c[4];
+            site_coords(e.L, s, c);
+            flow_link_pf(e.L, f, T + e.off / 72 * 48, wout + e.off, acc + e.acc_off, s, c, mu, eps, ca, cb, reunit);
+        }
+    }
+}
+
 // OBS = 0: sum of ReTr over all 6V plaquettes; OBS = 1: sum of -ReTr F_i^2 (magnetic)
 template <typename R, bool BUF, int OBS>
 __global__ void __launch_bounds__(kBS) ens_sum_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
@@ -138,12 +177,19 @@ void with_real(int is_f32, bool buf, Fn&& fn) {
 
 hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* win,
                                  int is_f32, void* wout, double* acc, double eps, double ca, double cb, int reunit,
-                                 uint32_t grid, hipStream_t st) {
+                                 uint32_t grid, void* T, hipStream_t st) {
     with_real(is_f32, fits_buffer(maxV, is_f32), [&](auto r, auto buf) {
         using R = decltype(r);
-        ens_flow_kernel<R, decltype(buf)::value><<<grid, kBS, 0, st>>>(tab, n, total, (4u * total + grid - 1) / grid,
-                                                                      (const R*)win, (R*)wout, acc, eps, ca, cb,
-                                                                      reunit);
+        constexpr bool B = decltype(buf)::value;
+        const uint32_t chunk4 = (4u * total + grid - 1) / grid;
+        if (T) {
+            ens_plaq_ta_kernel<R, B><<<grid, kBS, 0, st>>>(tab, n, total, (total + grid - 1) / grid, (const R*)win, (R*)T);
+            ens_flow_pf_kernel<R, B><<<grid, kBS, 0, st>>>(tab, n, total, chunk4, (const R*)win, (const R*)T, (R*)wout,
+                                                           acc, eps, ca, cb, reunit);
+        } else {
+            ens_flow_kernel<R, B><<<grid, kBS, 0, st>>>(tab, n, total, chunk4, (const R*)win, (R*)wout, acc, eps, ca,
+                                                        cb, reunit);
+        }
     });
     return hipGetLastError();
 }

@@ -53,8 +53,9 @@ class Ensemble:
         return GaugeField(self.dims[c], self.precision, self.device, _data=self.data[lo:hi])
 
     def autotune(self, reps=3):
-        """Time the persistent-grid variants of the ensemble flow on this ensemble (one flow
-        stage of every field, into scratch; the fields are only read) and remember the
+        """Time the variants of the ensemble flow on this ensemble (persistent grid of 1, 2,
+        4 or 8 workgroups per CU, with the staple or the plaquette-first stencil; one flow
+        stage of every field into scratch, the fields are only read) and remember the
         fastest for flow() on ensembles of this shape.  Returns (variant, ms per stage)."""
         best = ctypes.c_int32()
         ms = ctypes.c_double()

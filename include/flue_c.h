@@ -369,7 +369,8 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
 /*                                in storage order or (z, t) bricks 1x2, 2x2, 1x4, 2x4, 4x4   */
 /*                                (1..6, bit-identical), or plaquette-first (7, agrees to     */
 /*                                rounding; unitary links, see flue_wilson_grad_ex)           */
-/*   FLUE_OP_ENSEMBLE_FLOW  1..4  persistent grid of 1, 2, 4, 8 workgroups per CU            */
+/*   FLUE_OP_ENSEMBLE_FLOW  1..8  persistent grid of 1, 2, 4, 8 workgroups per CU with the   */
+/*                                staple stencil (1..4) or the plaquette-first one (5..8)    */
 /*                                (flue_ensemble_flow; tuned by flue_ensemble_autotune)      */
 /* ------------------------------------------------------------------ */
 enum { FLUE_OP_MAGNETIC = 0, FLUE_OP_B2_GRAD = 1, FLUE_OP_PLAQUETTE = 2, FLUE_OP_FLOW = 3, FLUE_OP_ENSEMBLE_FLOW = 4 };

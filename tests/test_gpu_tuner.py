@@ -153,18 +153,26 @@ def test_ensemble_grids_bit_identical(FL, tmp_path):
     seeds = [4100 + c for c in range(4)]
     nblk = sum((int(np.prod(d)) + 255) // 256 for d in dims)
     key = (4, nblk, max(int(np.prod(d)) for d in dims), 0)
-    ref = None
-    for v in (1, 2, 3, 4):
+    # grids 1..4 (staple sums) are bit-identical, as are 5..8 (plaquette-first); the two
+    # stencils agree to rounding and each matches the oracle's flow
+    ref = {}
+    for v in range(1, 9):
         force(FL, tmp_path, "ensemble_flow", key, 0, v)
         e = Ensemble.random(dims, seeds, 0.4, "fp64")
         plaq, B = e.flow(0.01, 4, measure_every=2, use_graph=True)
-        if ref is None:
-            ref = (e.data.clone(), plaq, B)
+        form = (v - 1) // 4
+        if form not in ref:
+            ref[form] = (e.data.clone(), plaq, B)
+            for c in (0, 2):
+                W = O.wilson_flow(O.random_field(dims[c], seeds[c], 0.4), 0.01, 4)
+                assert np.abs(e.field(c).to_complex().cpu().numpy() - W).max() < 1e-12, (v, c)
         else:
-            assert torch.equal(e.data, ref[0]), v
-            assert np.array_equal(plaq, ref[1]) and np.array_equal(B, ref[2]), v
+            assert torch.equal(e.data, ref[form][0]), v
+            assert np.array_equal(plaq, ref[form][1]) and np.array_equal(B, ref[form][2]), v
+    assert float((ref[0][0] - ref[1][0]).abs().max()) < 1e-12
+    assert np.abs(ref[0][2] - ref[1][2]).max() < 1e-12 * np.abs(ref[0][2]).max()
     e = Ensemble.random(dims, seeds, 0.4, "fp64")
     b, ms = e.autotune(reps=2)
-    assert 1 <= b <= 4 and ms > 0
+    assert 1 <= b <= 8 and ms > 0
     plaq, B = e.flow(0.01, 4, measure_every=2, use_graph=True)
-    assert torch.equal(e.data, ref[0]) and np.array_equal(B, ref[2])
+    assert float((e.data - ref[0][0]).abs().max()) < 1e-12
