@@ -760,10 +760,10 @@ int flue_wilson_grad_ex(const int32_t dims[4], const void* U, int32_t prec, doub
     const Lat L = make_lat(dims);
     if (variant == 0) variant = variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault);
     // the plaquette-first form needs unitary links (FP32 and declared-SU(3) fields) and a
-    // workspace; otherwise the staple form in storage order
+    // workspace; otherwise the staple form in the same visit order
     void* T = nullptr;
-    if (variant == kFlowPF || variant == kFlowPFWalk) {
-        if (prec == FLUE_F64 || !ws) variant = kFlowDefault;
+    if (flow_is_pf(variant)) {
+        if (prec == FLUE_F64 || !ws) variant = staple_order_of(variant);
         else if (ws_bytes < plaq_ta_bytes(L, prec == FLUE_F32)) return FLUE_EWS;
         else T = ws;
     }
@@ -808,8 +808,7 @@ int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* 
     if (check_prec(prec) || stage < 0 || stage > 2 || !std::isfinite(eps)) return FLUE_EINVAL;
     // Luescher RK3: acc <- ca acc + cb Z, (ca, cb) = (0, 1/4), (-17/9, 8/9), (-1, 3/4)
     // a single stage has no workspace for the plaquette-first form: staple sums
-    int variant = variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault);
-    if (variant == kFlowPF || variant == kFlowPFWalk) variant = kFlowDefault;
+    const int variant = staple_order_of(variant_or(FLUE_OP_FLOW, dims, prec, kFlowDefault));
     return flow_stage_run(make_lat(dims), Win, prec, Wout, acc, eps, stage, variant, nullptr, S(stream));
 }
 

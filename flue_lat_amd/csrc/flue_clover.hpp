@@ -142,18 +142,28 @@ __device__ __forceinline__ Mat<typename FV::real> first_link(const Quad<FV>& q) 
     }
 }
 
-// M * (link along d1 or d2 leaving P(a1, a2)); with C2 only rows 0 and 1 are formed
-template <int S, int AX, int a1, int a2, bool C2 = false, typename FV>
-__device__ __forceinline__ Mat<typename FV::real> ext(const Quad<FV>& q, const Mat<typename FV::real>& M) {
-    using R = typename FV::real;
+// the stored link that ext<S, AX, a1, a2> multiplies by (daggered in the product for S < 0)
+template <int S, int AX, int a1, int a2, typename FV>
+__device__ __forceinline__ Mat<typename FV::real> ext_link(const Quad<FV>& q) {
     const int ax = AX == 1 ? q.ax1 : q.ax2;
     constexpr int b1 = (AX == 1 && S < 0) ? a1 + 1 : a1;
     constexpr int b2 = (AX == 2 && S < 0) ? a2 + 1 : a2;
-    const Mat<R> U = load_link(q.f, ax, q.sp + q.T1[b1] + q.T2[b2]);
+    return load_link(q.f, ax, q.sp + q.T1[b1] + q.T2[b2]);
+}
+
+// M * U (S > 0) or M * U^+ (S < 0) for a link U from ext_link
+template <int S, bool C2, typename R>
+__device__ __forceinline__ Mat<R> ext_by(const Mat<R>& M, const Mat<R>& U) {
     if constexpr (S > 0)
         return C2 ? mul2(M, U) : mul(M, U);
     else
         return C2 ? mul2_nd(M, U) : mul_nd(M, U);
+}
+
+// M * (link along d1 or d2 leaving P(a1, a2)); with C2 only rows 0 and 1 are formed
+template <int S, int AX, int a1, int a2, bool C2 = false, typename FV>
+__device__ __forceinline__ Mat<typename FV::real> ext(const Quad<FV>& q, const Mat<typename FV::real>& M) {
+    return ext_by<S, C2>(M, ext_link<S, AX, a1, a2>(q));
 }
 
 // acc_ah on chain ends; with C2 the chains carry rows 0 and 1 and row 2 is rebuilt here
@@ -248,6 +258,102 @@ __device__ __forceinline__ void quadrant(const Quad<FV>& q, AH<typename FV::real
         CB3 = ext<S1, 1, 2, 3, C2>(q, CB3);             // CB(3,3)
         acc_leaf<C2>(h, w33, CA, CB3);                    // W(3,3)
     }
+    FLUE_SCHED_FENCE();
+}
+
+// quadrant() (non-clover) with a one-link lookahead: each group of products also loads the
+// link that the next group's first product needs, so that product starts on a link already
+// in registers while the group's other links are loaded.  (The scheduler otherwise issues
+// each link's loads just before the product that uses it.)  The products and their order are
+// those of quadrant(), so the results are bit-identical.  The FP64 forms at one wave per SIMD
+// use it: there no other wave covers the load latency, and the registers hold one more link.
+template <int S1, int S2, bool IN, bool OUT, bool C2, typename FV>
+__device__ __forceinline__ void quadrant_la(const Quad<FV>& q, AH<typename FV::real>& h,
+                                            const Lines<typename FV::real>* in = nullptr,
+                                            Lines<typename FV::real>* out = nullptr) {
+    using R = typename FV::real;
+    const R w11 = R(kK1), w22 = R(kK2), w33 = R(kK5), w12 = R(0.5 * kK3), w13 = R(0.5 * kK4);
+    // row a = 1
+    const Mat<R> L1 = first_link<S1, 1, 0, 0>(q);               // Line(x,d1,1)
+    Mat<R> Ld = first_link<S2, 2, 0, 0>(q);                    // Line(x,d2,1)
+    Mat<R> U = ext_link<S2, 2, 1, 0>(q), V = ext_link<S1, 1, 0, 1>(q);
+    Mat<R> P = ext_link<S2, 2, 1, 1>(q), Pn;
+    Mat<R> CA = ext_by<S2, C2>(L1, U);                               // CA(1,1)
+    Mat<R> CB1 = ext_by<S1, C2>(Ld, V);                              // CB(1,1)
+    acc_leaf<C2>(h, w11, CA, CB1);                                   // W(1,1)
+    FLUE_SCHED_FENCE();
+    U = ext_link<S2, 2, 0, 1>(q);
+    V = ext_link<S1, 1, 0, 2>(q);
+    Pn = ext_link<S2, 2, 1, 2>(q);
+    CA = ext_by<S2, C2>(CA, P);                                      // CA(1,2)
+    Ld = ext_by<S2, C2>(Ld, U);                                      // Line(x,d2,2)
+    if constexpr (OUT) out->l2 = Ld;
+    Mat<R> CB2 = ext_by<S1, C2>(Ld, V);                              // CB(2,1)
+    acc_leaf<C2>(h, w12, CA, CB2);                                   // W(1,2)
+    P = Pn;
+    FLUE_SCHED_FENCE();
+    U = ext_link<S2, 2, 0, 2>(q);
+    V = ext_link<S1, 1, 0, 3>(q);
+    if constexpr (IN)
+        Pn = ext_link<S2, 2, 2, 0>(q);
+    else
+        Pn = ext_link<S1, 1, 1, 0>(q);
+    CA = ext_by<S2, C2>(CA, P);                                      // CA(1,3)
+    Ld = ext_by<S2, C2>(Ld, U);                                      // Line(x,d2,3)
+    if constexpr (OUT) out->l3 = Ld;
+    Mat<R> CB3 = ext_by<S1, C2>(Ld, V);                              // CB(3,1)
+    acc_leaf<C2>(h, w13, CA, CB3);                                   // W(1,3)
+    P = Pn;
+    FLUE_SCHED_FENCE();
+    // row a = 2
+    if constexpr (!IN) U = ext_link<S2, 2, 2, 0>(q);
+    V = ext_link<S1, 1, 1, 1>(q);
+    Pn = ext_link<S2, 2, 2, 1>(q);
+    Mat<R> L;
+    if constexpr (IN) {
+        L = in->l2;                                                  // Line(x,d1,2)
+        CA = ext_by<S2, C2>(L, P);                                   // CA(2,1)
+    } else {
+        L = ext_by<S1, C2>(L1, P);
+        CA = ext_by<S2, C2>(L, U);
+    }
+    CB1 = ext_by<S1, C2>(CB1, V);                                    // CB(1,2)
+    acc_leaf<C2>(h, w12, CA, CB1);                                   // W(2,1)
+    P = Pn;
+    FLUE_SCHED_FENCE();
+    V = ext_link<S1, 1, 1, 2>(q);
+    if constexpr (IN)
+        Pn = ext_link<S2, 2, 3, 0>(q);
+    else
+        Pn = ext_link<S1, 1, 2, 0>(q);
+    CA = ext_by<S2, C2>(CA, P);                                      // CA(2,2)
+    CB2 = ext_by<S1, C2>(CB2, V);                                    // CB(2,2)
+    acc_leaf<C2>(h, w22, CA, CB2);                                   // W(2,2)
+    P = Pn;
+    FLUE_SCHED_FENCE();
+    // row a = 3
+    if constexpr (!IN) U = ext_link<S2, 2, 3, 0>(q);
+    V = ext_link<S1, 1, 2, 1>(q);
+    Pn = ext_link<S2, 2, 3, 1>(q);
+    if constexpr (IN) {
+        L = in->l3;                                                  // Line(x,d1,3)
+        CA = ext_by<S2, C2>(L, P);                                   // CA(3,1)
+    } else {
+        L = ext_by<S1, C2>(L, P);
+        CA = ext_by<S2, C2>(L, U);
+    }
+    CB1 = ext_by<S1, C2>(CB1, V);                                    // CB(1,3)
+    acc_leaf<C2>(h, w13, CA, CB1);                                   // W(3,1)
+    P = Pn;
+    FLUE_SCHED_FENCE();
+    U = ext_link<S2, 2, 3, 2>(q);
+    V = ext_link<S1, 1, 1, 3>(q);
+    Pn = ext_link<S1, 1, 2, 3>(q);
+    CA = ext_by<S2, C2>(CA, P);                                      // CA(3,2)
+    CA = ext_by<S2, C2>(CA, U);                                      // CA(3,3)
+    CB3 = ext_by<S1, C2>(CB3, V);                                    // CB(3,2)
+    CB3 = ext_by<S1, C2>(CB3, Pn);                                   // CB(3,3)
+    acc_leaf<C2>(h, w33, CA, CB3);                                   // W(3,3)
     FLUE_SCHED_FENCE();
 }
 
@@ -350,13 +456,20 @@ __device__ __forceinline__ AH<typename FV::real> plane_ah_parked(const PlaneGeom
 
 // the four quadrants in counter-clockwise order: (+mu,+nu), (+nu,-mu), (-mu,-nu), (-nu,+mu).
 // Each quadrant's d2 is the next one's d1; with SHARE the lines along it are computed once.
-// With C2 the link chains carry two rows (SU(3) links only; see mul2).
-template <bool CLOVER, bool SHARE = false, bool C2 = false, typename FV>
+// With C2 the link chains carry two rows (SU(3) links only; see mul2).  LA (with SHARE) takes
+// the quadrants in their lookahead form, quadrant_la.
+template <bool CLOVER, bool SHARE = false, bool C2 = false, bool LA = false, typename FV>
 __device__ __forceinline__ AH<typename FV::real> plane_ah(const PlaneGeom<FV>& g) {
     using R = typename FV::real;
     AH<R> h;
     ah_zero(h);
-    if constexpr (SHARE && !CLOVER) {
+    if constexpr (LA && SHARE && !CLOVER) {
+        Lines<R> a, b;
+        quadrant_la<1, 1, false, true, C2>(make_quad<1, 1, true>(g), h, nullptr, &a);
+        quadrant_la<1, -1, true, true, C2>(make_quad<1, -1, false>(g), h, &a, &b);
+        quadrant_la<-1, -1, true, true, C2>(make_quad<-1, -1, true>(g), h, &b, &a);
+        quadrant_la<-1, 1, true, false, C2>(make_quad<-1, 1, false>(g), h, &a, nullptr);
+    } else if constexpr (SHARE && !CLOVER) {
         Lines<R> a, b;
         quadrant<1, 1, false, false, true, C2>(make_quad<1, 1, true>(g), h, nullptr, &a);
         quadrant<1, -1, false, true, true, C2>(make_quad<1, -1, false>(g), h, &a, &b);

@@ -85,11 +85,18 @@ uint32_t grad_blocks(const Lat& L);
 // stencil variants (k_grad.hip): 1 = staple sums in storage order, 2..6 = staple sums in
 // (z, t) brick orders, 7 = plaquette-first (the six plaquette parts of every site into T,
 // plaq_ta_bytes, then each link from 3 links and 6 stored parts; unitary links only),
-// 8 = plaquette-first with both passes walking along t
-constexpr int kFlowVariants = 8;
+// 8 = plaquette-first with both passes walking along t, 9..13 = plaquette-first in the (z, t)
+// brick orders of 2..6
+constexpr int kFlowVariants = 13;
 constexpr int kFlowPF = 7;
 constexpr int kFlowPFWalk = 8;
 constexpr int kFlowDefault = 1;
+// whether a flow/gradient variant is a plaquette-first form (needs plaq_ta_bytes of workspace)
+constexpr bool flow_is_pf(int variant) { return variant >= kFlowPF && variant <= kFlowVariants; }
+// the staple-sum variant with the visit order of `variant` (storage order for 7 and 8)
+constexpr int staple_order_of(int variant) {
+    return variant > kFlowPFWalk ? variant - kFlowPF : (flow_is_pf(variant) ? kFlowDefault : variant);
+}
 size_t plaq_ta_bytes(const Lat& L, int is_f32);
 hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st,
                               int variant, void* T);

@@ -71,6 +71,8 @@ magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ pa
     // FP64 at two waves per SIMD: the waiting chains are parked in LDS (quadrant_parked)
     constexpr bool kPark = sizeof(typename FV::real) == 8 && WPE == 2;
     constexpr bool kShare = !kPark && (kC2 || WPE == 1);
+    // FP64 at one wave per SIMD: each group of products loads the next group's first link
+    constexpr bool kLA = sizeof(typename FV::real) == 8 && WPE == 1;
     // the plane comes from blockIdx.y so that it, and every buffer descriptor derived
     // from it, stays in scalar registers (a division of blockIdx.x would not)
     const int pl = (int)blockIdx.y;
@@ -97,13 +99,13 @@ magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ pa
         } else if constexpr (PLANE_MAJOR) {
             const uint32_t s = plane_site(L, po, pl, o, c);
             const int mu = plane_mu(pl), nu = plane_nu(pl);
-            acc = f_norm2(plane_ah<false, kShare, kC2>(plane_geom(L, f, s, c[mu], c[nu], mu, nu)));
+            acc = f_norm2(plane_ah<false, kShare, kC2, kLA>(plane_geom(L, f, s, c[mu], c[nu], mu, nu)));
         } else {
             site_coords(L, o, c);
 #pragma unroll 1
             for (int p = 0; p < 3; ++p) {
                 const int mu = plane_mu(p), nu = plane_nu(p);
-                acc += f_norm2(plane_ah<false, kShare, kC2>(plane_geom(L, f, o, c[mu], c[nu], mu, nu)));
+                acc += f_norm2(plane_ah<false, kShare, kC2, kLA>(plane_geom(L, f, o, c[mu], c[nu], mu, nu)));
             }
         }
     }

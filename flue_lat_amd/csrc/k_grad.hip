@@ -77,19 +77,20 @@ __global__ void __launch_bounds__(kBS) flow_stage_kernel(Lat L, FV f, BlockOrder
 
 // plaquette-first stencil (kFlowPF): the plaquette parts of every site, then per link
 template <typename FV>
-__global__ void __launch_bounds__(kBS) plaq_ta_kernel(Lat L, FV f, typename FV::real* __restrict__ T) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+__global__ void __launch_bounds__(kBS) plaq_ta_kernel(Lat L, FV f, BlockOrder bo, typename FV::real* __restrict__ T) {
+    const uint32_t s = site_block(L, bo, xcd_block(blockIdx.x, gridDim.x)) * kBS + threadIdx.x;
     if (s >= L.V) return;
     plaquette_ta_site(L, f, s, T);
 }
 
 template <typename FV>
-__global__ void __launch_bounds__(kBS) wilson_grad_pf_kernel(Lat L, FV f, const typename FV::real* __restrict__ T,
-                                                             double beta, typename FV::real* __restrict__ grad) {
+__global__ void __launch_bounds__(kBS) wilson_grad_pf_kernel(Lat L, FV f, BlockOrder bo,
+                                                             const typename FV::real* __restrict__ T, double beta,
+                                                             typename FV::real* __restrict__ grad) {
     using R = typename FV::real;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int mu = (int)(blk & 3u);
-    const uint32_t s = (blk >> 2) * kBS + threadIdx.x;
+    const uint32_t s = site_block(L, bo, blk >> 2) * kBS + threadIdx.x;
     if (s >= L.V) return;
     int c[4];
     site_coords(L, s, c);
@@ -104,13 +105,14 @@ __global__ void __launch_bounds__(kBS) wilson_grad_pf_kernel(Lat L, FV f, const 
 }
 
 template <typename FV>
-__global__ void __launch_bounds__(kBS) flow_stage_pf_kernel(Lat L, FV f, const typename FV::real* __restrict__ T,
+__global__ void __launch_bounds__(kBS) flow_stage_pf_kernel(Lat L, FV f, BlockOrder bo,
+                                                            const typename FV::real* __restrict__ T,
                                                             typename FV::real* __restrict__ wout,
                                                             double* __restrict__ acc, double eps, double ca, double cb,
                                                             int reunit) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int mu = (int)(blk & 3u);
-    const uint32_t s = (blk >> 2) * kBS + threadIdx.x;
+    const uint32_t s = site_block(L, bo, blk >> 2) * kBS + threadIdx.x;
     if (s >= L.V) return;
     int c[4];
     site_coords(L, s, c);
@@ -209,10 +211,13 @@ constexpr uint32_t kBrickZ[kFlowPF - 1] = {0, 1, 2, 1, 2, 4};  // variants 1..6
 constexpr uint32_t kBrickT[kFlowPF - 1] = {0, 2, 2, 4, 4, 4};
 
 // the order of a variant, or the storage order where the bricks do not tile the lattice
-// (planes of a whole number of blocks, extents divisible by the brick)
+// (planes of a whole number of blocks, extents divisible by the brick).  Plaquette-first
+// variant kFlowPF + k (k = 2..6) takes the order of staple variant k; kFlowPF and the walks
+// (which order their own blocks) take the storage order.
 static BlockOrder block_order(const Lat& L, int variant) {
     BlockOrder bo{0, 0, 1, 1};
-    if (variant < 1 || variant >= kFlowPF) variant = kFlowDefault;  // the plaquette-first forms order their own blocks
+    if (variant > kFlowPFWalk && variant <= kFlowVariants) variant -= kFlowPF;
+    if (variant < 1 || variant >= kFlowPF) variant = kFlowDefault;
     const uint32_t bz = kBrickZ[variant - 1], bt = kBrickT[variant - 1];
     const uint32_t plane = (uint32_t)L.n[1] * (uint32_t)L.n[2];
     if (bz == 0 || plane % kBS != 0 || L.n[3] % bz != 0 || L.n[0] % bt != 0) return bo;
@@ -227,7 +232,7 @@ size_t plaq_ta_bytes(const Lat& L, int is_f32) { return (size_t)48 * L.V * (is_f
 hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st,
                               int variant, void* T) {
     const uint32_t g = grad_blocks(L);
-    if ((variant == kFlowPF || variant == kFlowPFWalk) && !T) return hipErrorInvalidValue;
+    if (flow_is_pf(variant) && !T) return hipErrorInvalidValue;
     const BlockOrder bo = block_order(L, variant);
     const uint32_t nbs = (L.Vs + kBS - 1) / kBS, ntc = ((uint32_t)L.n[0] + kPFWalk - 1) / kPFWalk;
     with_view(L.V, U, is_f32, [&](auto f) {
@@ -237,9 +242,9 @@ hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double be
             plaq_ta_walk_kernel<F><<<nbs * ntc, kBS, 0, st>>>(L, f, nbs, (R*)T);
             pf_walk_kernel<true, F><<<4 * nbs * ntc, kBS, 0, st>>>(L, f, (const R*)T, nbs, (R*)grad, nullptr, 0.0, 0.0,
                                                                   0.0, 0, beta);
-        } else if (variant == kFlowPF) {
-            plaq_ta_kernel<F><<<(L.V + kBS - 1) / kBS, kBS, 0, st>>>(L, f, (R*)T);
-            wilson_grad_pf_kernel<F><<<g, kBS, 0, st>>>(L, f, (const R*)T, beta, (R*)grad);
+        } else if (flow_is_pf(variant)) {
+            plaq_ta_kernel<F><<<(L.V + kBS - 1) / kBS, kBS, 0, st>>>(L, f, bo, (R*)T);
+            wilson_grad_pf_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, (const R*)T, beta, (R*)grad);
         } else {
             wilson_grad_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, beta, (R*)grad);
         }
@@ -250,7 +255,7 @@ hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double be
 hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, double* acc, double eps,
                              double ca, double cb, int reunit, hipStream_t st, int variant, void* T) {
     const uint32_t g = grad_blocks(L);
-    if ((variant == kFlowPF || variant == kFlowPFWalk) && !T) return hipErrorInvalidValue;
+    if (flow_is_pf(variant) && !T) return hipErrorInvalidValue;
     const BlockOrder bo = block_order(L, variant);
     const uint32_t nbs = (L.Vs + kBS - 1) / kBS, ntc = ((uint32_t)L.n[0] + kPFWalk - 1) / kPFWalk;
     with_view(L.V, Win, is_f32, [&](auto f) {
@@ -260,9 +265,9 @@ hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wo
             plaq_ta_walk_kernel<F><<<nbs * ntc, kBS, 0, st>>>(L, f, nbs, (R*)T);
             pf_walk_kernel<false, F><<<4 * nbs * ntc, kBS, 0, st>>>(L, f, (const R*)T, nbs, (R*)Wout, acc, eps, ca, cb,
                                                                    reunit, 0.0);
-        } else if (variant == kFlowPF) {
-            plaq_ta_kernel<F><<<(L.V + kBS - 1) / kBS, kBS, 0, st>>>(L, f, (R*)T);
-            flow_stage_pf_kernel<F><<<g, kBS, 0, st>>>(L, f, (const R*)T, (R*)Wout, acc, eps, ca, cb, reunit);
+        } else if (flow_is_pf(variant)) {
+            plaq_ta_kernel<F><<<(L.V + kBS - 1) / kBS, kBS, 0, st>>>(L, f, bo, (R*)T);
+            flow_stage_pf_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, (const R*)T, (R*)Wout, acc, eps, ca, cb, reunit);
         } else {
             flow_stage_kernel<F><<<g, kBS, 0, st>>>(L, f, bo, (R*)Wout, acc, eps, ca, cb, reunit);
         }

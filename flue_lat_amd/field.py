@@ -339,8 +339,9 @@ class GaugeField:
     def wilson_grad(self, beta, variant=0):
         """Lie gradient of the Wilson action, G = (beta/6) P_TA(U Sigma), as a GaugeField-shaped
         su(3) field (same SoA layout and precision).  variant: the stencil (0 = tuned; 1..6
-        staple sums in the autotuner's visit orders; 7 plaquette-first, used for FP32 and
-        "fp64-su3" fields, whose links are SU(3))."""
+        staple sums in the autotuner's visit orders; 7..13 plaquette-first in storage order,
+        walking along t, or in the brick orders of 2..6, used for FP32 and "fp64-su3" fields,
+        whose links are SU(3))."""
         g = self._like()
         n = _lib.lib().flue_pf_work_bytes(self._cdims, self.prec)
         ws = torch.empty(n, dtype=torch.uint8, device=self.device)

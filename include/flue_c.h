@@ -314,12 +314,13 @@ int flue_wilson_grad(const int32_t dims[4], const void* U, int32_t prec, double 
 size_t flue_pf_work_bytes(const int32_t dims[4], int32_t prec);
 
 /* flue_wilson_grad with an explicit FLUE_OP_FLOW variant (0 = tuned): 1..6 staple sums in
- * the visit orders of the autotuner table, 7 = plaquette-first: the six plaquettes of every
+ * the visit orders of the autotuner table, 7..13 plaquette-first: the six plaquettes of every
  * site are formed once and their traceless anti-Hermitian parts stored in ws
  * (flue_pf_work_bytes), then each link takes its up plaquettes at x and its down plaquettes
- * at x - nu conjugated by U_nu(x - nu).  That identity holds for unitary links, so variant 7
- * runs for FLUE_F32 and FLUE_F64_SU3 fields given a workspace; FLUE_F64 fields and ws == NULL
- * take variant 1. */
+ * at x - nu conjugated by U_nu(x - nu) (7 in storage order, 8 walking along t, 9..13 in the
+ * orders of 2..6).  That identity holds for unitary links, so these variants run for FLUE_F32
+ * and FLUE_F64_SU3 fields given a workspace; FLUE_F64 fields and ws == NULL take the staple
+ * sums in the same visit order (variant 1 for 7 and 8). */
 int flue_wilson_grad_ex(const int32_t dims[4], const void* U, int32_t prec, double beta, int32_t variant, void* grad,
                         void* ws, size_t ws_bytes, void* stream);
 
@@ -349,8 +350,9 @@ int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* 
  * replays it (needs a non-default stream); `work` is flue_flow_work_bytes (NULL = library
  * buffer, which must then exist before any capture).  The flow moves SU(3) links along the
  * group (exponential map, reunitarisation after every step), so it takes SU(3) links in
- * every precision: the tuned FLUE_OP_FLOW variant, including the plaquette-first variant 7,
- * is used.  flue_flow_stage (no workspace) uses the staple forms only. */
+ * every precision: the tuned FLUE_OP_FLOW variant, including the plaquette-first variants
+ * 7..13, is used.  flue_flow_stage (no workspace) uses the staple sums in the tuned
+ * variant's visit order. */
 int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, int32_t nsteps, int32_t use_graph,
                      void* work, size_t work_bytes, void* stream);
 
@@ -365,10 +367,12 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
 /*   FLUE_OP_B2_GRAD        1..2  per-link gather / reverse mode with slots (flue_b2_grad_ex) */
 /*   FLUE_OP_PLAQUETTE      1..6  time slices per thread 2, 3, 4, 6, 8, 12 (flue_plaquette,   */
 /*                                flue_wilson_action)                                        */
-/*   FLUE_OP_FLOW           1..7  stencil of the flow and the Wilson gradient: staple sums   */
+/*   FLUE_OP_FLOW          1..13  stencil of the flow and the Wilson gradient: staple sums   */
 /*                                in storage order or (z, t) bricks 1x2, 2x2, 1x4, 2x4, 4x4   */
-/*                                (1..6, bit-identical), or plaquette-first (7, agrees to     */
-/*                                rounding; unitary links, see flue_wilson_grad_ex)           */
+/*                                (1..6, bit-identical); plaquette-first (unitary links, see  */
+/*                                flue_wilson_grad_ex) in storage order (7), walking along t  */
+/*                                (8) or in the same five brick orders (9..13); 7 and 9..13   */
+/*                                are bit-identical, the two stencils agree to rounding       */
 /*   FLUE_OP_ENSEMBLE_FLOW  1..8  persistent grid of 1, 2, 4, 8 workgroups per CU with the   */
 /*                                staple stencil (1..4) or the plaquette-first one (5..8)    */
 /*                                (flue_ensemble_flow; tuned by flue_ensemble_autotune)      */
