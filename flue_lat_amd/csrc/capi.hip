@@ -937,8 +937,9 @@ size_t flue_autotune_work_bytes(int32_t op, const int32_t dims[4], int32_t prec)
     case FLUE_OP_MAGNETIC: return partial_bytes(magnetic_blocks(L, prec == FLUE_F32));
     case FLUE_OP_PLAQUETTE: return partial_bytes(plaquette_max_blocks(L));
     case FLUE_OP_B2_GRAD: return flue_b2_work_bytes(dims, prec) + fb;  // work, then a gradient
-    case FLUE_OP_FLOW:  // output links, accumulator, plaquette parts
-        return flow_acc_offset(L, prec) + (size_t)L.V * 32 * sizeof(double) + plaq_ta_bytes(L, prec == FLUE_F32);
+    case FLUE_OP_FLOW:  // two link buffers, accumulator, plaquette parts
+        return flow_acc_offset(L, prec) + (size_t)L.V * 32 * sizeof(double) + plaq_ta_bytes(L, prec == FLUE_F32) +
+               flow_acc_offset(L, prec);
     default: return 0;
     }
 }
@@ -967,10 +968,16 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
         case FLUE_OP_MAGNETIC: return hip_status(launch_magnetic(L, U, is_f32, su3, (double*)w, st, v, nullptr));
         case FLUE_OP_PLAQUETTE: return hip_status(launch_plaquette(L, U, is_f32, 0, 3, 3, (double*)w, st, v, nullptr));
         case FLUE_OP_B2_GRAD: return b2_grad_run(L, U, prec, v, w, (char*)w + flue_b2_work_bytes(dims, prec), st);
-        default:  // FLUE_OP_FLOW: the first RK3 stage into scratch links.  It only writes the
-                  // accumulator: a later stage would scale it by -17/9 or -1 on every timed
-                  // call, and the growing exponent would slow each variant more than the last.
-            return flow_stage_run(L, U, prec, w, facc, 0.01, 0, v, (char*)facc + (size_t)L.V * 32 * 8, st);
+        default: {  // FLUE_OP_FLOW: one RK3 step, U -> w -> w2 -> w.  Stage 0 sets the
+                    // accumulator afresh, so every timed step sees the same values (stages 1
+                    // and 2 alone would scale it on every call and slow each variant more).
+            void* T = (char*)facc + (size_t)L.V * 32 * 8;
+            void* w2 = (char*)T + plaq_ta_bytes(L, is_f32);
+            int r = flow_stage_run(L, U, prec, w, facc, 0.01, 0, v, T, st);
+            if (!r) r = flow_stage_run(L, w, prec, w2, facc, 0.01, 1, v, T, st);
+            if (!r) r = flow_stage_run(L, w2, prec, w, facc, 0.01, 2, v, T, st);
+            return r;
+        }
         }
     };
     int bv = 0;

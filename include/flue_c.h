@@ -381,9 +381,10 @@ enum { FLUE_OP_MAGNETIC = 0, FLUE_OP_B2_GRAD = 1, FLUE_OP_PLAQUETTE = 2, FLUE_OP
 /* device bytes flue_autotune needs as `ws` for op (0 for an invalid op) */
 size_t flue_autotune_work_bytes(int32_t op, const int32_t dims[4], int32_t prec);
 /* times every variant of op (not FLUE_OP_ENSEMBLE_FLOW) on U, reps calls each after a
- * warm-up; U is only read.  ws: flue_autotune_work_bytes(op, dims, prec) (NULL = library
- * buffer).  Waits for the stream.  With the environment variable FLUE_TUNE_LOG set, every
- * variant's time is printed to stderr. */
+ * warm-up (FLUE_OP_FLOW: one RK3 step into scratch links); U is only read.  ws:
+ * flue_autotune_work_bytes(op, dims, prec) (NULL = library buffer).  Waits for the stream.
+ * With the environment variable FLUE_TUNE_LOG set, every variant's time is printed to
+ * stderr. */
 int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec, int32_t reps, int32_t* best,
                   double* best_ms, void* ws, size_t ws_bytes, void* stream);
 int flue_tuned_variant(int32_t op, const int32_t dims[4], int32_t prec, int32_t* variant);
