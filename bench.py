@@ -274,6 +274,11 @@ def main():
             "frac": round(achieved_tflops / PEAK_TFLOPS["fp32"], 4),
             "traffic": traffic,
             "flop_per_site": FLOP_PER_SITE,
+            # the canonical count (28 leaves x 3 products per plane); the rectangle-first form
+            # (magnetic variant 7) executes fewer, so there frac is the fraction of the
+            # canonical algorithm's VALU ceiling rather than of the VALU issue rate
+            "flop_basis": "canonical",
+            "variant": tuned["fp32"],
             # the code is built without packed FP32 instructions: their rate is half the peak
             "peak_unpacked": PEAK_TFLOPS["fp32"] / 2,
             "frac_of_unpacked": round(achieved_tflops / (PEAK_TFLOPS["fp32"] / 2), 4),

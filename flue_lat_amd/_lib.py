@@ -88,6 +88,7 @@ SIGNATURES = {
     "flue_wilson_grad": [_P, _P, _I, _D, _P, _P],
     "flue_wilson_grad_ex": [_P, _P, _I, _D, _I, _P, _P, _SZ, _P],
     "flue_pf_work_bytes": [_P, _I],
+    "flue_magnetic_work_bytes": [_P, _I],
     "flue_b2_grad": [_P, _P, _I, _P, _P, _SZ, _P],
     "flue_b2_grad_ex": [_P, _P, _I, _I, _P, _P, _SZ, _P],
     "flue_flow_stage": [_P, _P, _I, _P, _P, _D, _I, _P],
@@ -106,7 +107,7 @@ SIGNATURES = {
 _RESTYPE = {"flue_strerror": ctypes.c_char_p, "flue_field_bytes": _SZ, "flue_workspace_bytes": _SZ,
             "flue_b2_work_bytes": _SZ, "flue_flow_work_bytes": _SZ, "flue_ensemble_field_bytes": _SZ,
             "flue_ensemble_work_bytes": _SZ, "flue_autotune_work_bytes": _SZ,
-            "flue_pf_work_bytes": _SZ}
+            "flue_pf_work_bytes": _SZ, "flue_magnetic_work_bytes": _SZ}
 
 
 def lib():

@@ -87,6 +87,30 @@ size_t partial_bytes(uint32_t nblocks) {
     return ((size_t)nblocks + (nblocks + kFinalChunk - 1) / kFinalChunk) * 2 * sizeof(double) + 256;
 }
 
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// whether the rectangle-first magnetic form (variant 7) applies to the lattice
+bool magnetic_rect_ok(const Lat& L, int32_t prec) { return rect_fits(L) && fits_buffer(L.V, prec == FLUE_F32); }
+
+// magnetic workspace of a variant: the partials, and for variant 7 the footprints after them
+size_t magnetic_rect_partials(const Lat& L, int32_t prec) {
+    return align256(partial_bytes(2 * rect_tiles(L) + magnetic_blocks(L, prec == FLUE_F32) / 3));
+}
+size_t magnetic_ws_bytes(const Lat& L, int32_t prec, int variant) {
+    if (variant == kMagneticRect && magnetic_rect_ok(L, prec))
+        return magnetic_rect_partials(L, prec) + rect_fp_bytes(L, prec == FLUE_F32);
+    return partial_bytes(magnetic_blocks(L, prec == FLUE_F32));
+}
+
+// runs magnetic `variant` (1..7) on workspace w and finalizes B into d_B
+int magnetic_run(const Lat& L, const void* U, int32_t prec, int variant, void* w, double* d_B, hipStream_t st) {
+    uint32_t np = 0;
+    void* fp = variant == kMagneticRect ? (char*)w + magnetic_rect_partials(L, prec) : nullptr;
+    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, prec == FLUE_F64_SU3, (double*)w, st, variant, &np, fp));
+    if (d_B) FLUE_TRY(launch_finalize((double*)w, np, d_B, 1.0 / (double)L.V, st));
+    return FLUE_OK;
+}
+
 // ---------------------------------------------------------------- SU(3) check
 // per-block (count, max, max) triples and the combined triple
 size_t check_bytes(const Lat& L) { return ((size_t)check_blocks(L) * 3 + 3) * sizeof(double) + 256; }
@@ -636,22 +660,29 @@ int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_
     return flue_magnetic_ex(dims, U, prec, 0, d_B, ws, ws_bytes, stream);
 }
 
+size_t flue_magnetic_work_bytes(const int32_t dims[4], int32_t prec) {
+    if (check_dims(dims) || check_prec(prec)) return 0;
+    const Lat L = make_lat(dims);
+    return std::max(magnetic_ws_bytes(L, prec, kMagneticRect), magnetic_ws_bytes(L, prec, 1));
+}
+
 int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B, void* ws,
                      size_t ws_bytes, void* stream) {
     int rc = check_dims(dims);
     if (rc) return rc;
     if (!U || !d_B) return FLUE_ENULL;
-    if (check_prec(prec)) return FLUE_EINVAL;
+    if (check_prec(prec) || variant < 0 || variant > kMagneticVariants) return FLUE_EINVAL;
     const Lat L = make_lat(dims);
-    const uint32_t nb = magnetic_blocks(L, prec == FLUE_F32);
-    void* w;
-    if ((rc = get_ws(ws, ws_bytes, partial_bytes(nb), &w))) return rc;
-    if (variant < 0 || variant > kMagneticVariants) return FLUE_EINVAL;
     if (variant == 0) variant = tuned_variant(FLUE_OP_MAGNETIC, dims, prec);
-    uint32_t np = 0;
-    FLUE_TRY(launch_magnetic(L, U, prec == FLUE_F32, prec == FLUE_F64_SU3, (double*)w, S(stream), variant, &np));
-    FLUE_TRY(launch_finalize((double*)w, np, d_B, 1.0 / (double)L.V, S(stream)));
-    return FLUE_OK;
+    if (variant == 0) variant = magnetic_default_variant(prec == FLUE_F32);
+    // the rectangle-first form needs tileable extents and room for its footprints; otherwise
+    // the precision's default form
+    if (variant == kMagneticRect &&
+        (!magnetic_rect_ok(L, prec) || (ws && ws_bytes < magnetic_ws_bytes(L, prec, kMagneticRect))))
+        variant = magnetic_default_variant(prec == FLUE_F32);
+    void* w;
+    if ((rc = get_ws(ws, ws_bytes, magnetic_ws_bytes(L, prec, variant), &w))) return rc;
+    return magnetic_run(L, U, prec, variant, w, d_B, S(stream));
 }
 
 // ---------------------------------------------------------------- allocation-free magnetic B
@@ -934,7 +965,7 @@ size_t flue_autotune_work_bytes(int32_t op, const int32_t dims[4], int32_t prec)
     const Lat L = make_lat(dims);
     const size_t fb = flue_field_bytes(dims, prec);
     switch (op) {
-    case FLUE_OP_MAGNETIC: return partial_bytes(magnetic_blocks(L, prec == FLUE_F32));
+    case FLUE_OP_MAGNETIC: return flue_magnetic_work_bytes(dims, prec);
     case FLUE_OP_PLAQUETTE: return partial_bytes(plaquette_max_blocks(L));
     case FLUE_OP_B2_GRAD: return flue_b2_work_bytes(dims, prec) + fb;  // work, then a gradient
     case FLUE_OP_FLOW:  // two link buffers, accumulator, plaquette parts
@@ -951,7 +982,7 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
     if (!U) return FLUE_ENULL;
     if (check_prec(prec) || reps < 1 || op_variants(op) == 0 || op == FLUE_OP_ENSEMBLE_FLOW) return FLUE_EINVAL;
     const Lat L = make_lat(dims);
-    const int is_f32 = prec == FLUE_F32, su3 = prec == FLUE_F64_SU3;
+    const int is_f32 = prec == FLUE_F32;
     if (op == FLUE_OP_MAGNETIC && !fits_buffer(L.V, is_f32)) {
         // only one variant exists for fields beyond the buffer-resource range
         if (best) *best = magnetic_default_variant(is_f32);
@@ -965,7 +996,7 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
     double* facc = (double*)((char*)w + flow_acc_offset(L, prec));
     auto launch = [&](int v) -> int {
         switch (op) {
-        case FLUE_OP_MAGNETIC: return hip_status(launch_magnetic(L, U, is_f32, su3, (double*)w, st, v, nullptr));
+        case FLUE_OP_MAGNETIC: return magnetic_run(L, U, prec, v, w, nullptr, st);
         case FLUE_OP_PLAQUETTE: return hip_status(launch_plaquette(L, U, is_f32, 0, 3, 3, (double*)w, st, v, nullptr));
         case FLUE_OP_B2_GRAD: return b2_grad_run(L, U, prec, v, w, (char*)w + flue_b2_work_bytes(dims, prec), st);
         default: {  // FLUE_OP_FLOW: one RK3 step, U -> w -> w2 -> w.  Stage 0 sets the
@@ -982,7 +1013,9 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
     };
     int bv = 0;
     double bms = 0.0;
-    if ((rc = time_variants(op_variants(op), reps, st, launch, &bv, &bms))) return rc;
+    // the rectangle-first magnetic form (the last variant) only where it applies
+    const int nvar = op == FLUE_OP_MAGNETIC && !magnetic_rect_ok(L, prec) ? kMagneticRect - 1 : op_variants(op);
+    if ((rc = time_variants(nvar, reps, st, launch, &bv, &bms))) return rc;
     remember(op, dims, prec, bv, bms);
     if (best) *best = bv;
     if (best_ms) *best_ms = bms;
@@ -1063,8 +1096,6 @@ int ens_plan(int32_t n, const int32_t* dims, EnsPlan& p) {
     p.total_blocks = (uint32_t)blocks;
     return FLUE_OK;
 }
-
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // autotuner key of an ensemble: its size, logical blocks and largest volume
 void ens_key(const EnsPlan& p, int32_t n, int32_t key[4]) {

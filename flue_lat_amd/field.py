@@ -87,6 +87,7 @@ class GaugeField:
         self.data = _data
         self._cdims = _i32(*self.dims)
         self._ws = None
+        self._mws = None
         if check and self.prec == FLUE_F64_SU3 and declared:
             self.check_su3()
 
@@ -100,6 +101,14 @@ class GaugeField:
             n = _lib.lib().flue_workspace_bytes(self._cdims)
             self._ws = torch.empty(n, dtype=torch.uint8, device=self.device)
         return self._ws
+
+    def magnetic_workspace(self):
+        """The workspace of magnetic(): every variant's, including the rectangle-first
+        form's footprints (flue_magnetic_work_bytes); kept with the field."""
+        if self._mws is None:
+            n = _lib.lib().flue_magnetic_work_bytes(self._cdims, self.prec)
+            self._mws = torch.empty(n, dtype=torch.uint8, device=self.device)
+        return self._mws
 
     @classmethod
     def random(cls, dims, seed, eps, precision="fp64", device=None):
@@ -296,9 +305,10 @@ class GaugeField:
     def magnetic(self, sync=True, variant=0):
         """B = -(1/V) sum_x sum_i ReTr F_i^2 over the spatial planes (src/wloops.F90:479-544).
         variant selects the kernel form and occupancy target: 1-3 site-major, 4-6
-        plane-major, at 1-3 waves per SIMD (0 = tuned or default choice)."""
+        plane-major, at 1-3 waves per SIMD; 7 rectangle-first (NX, NY, NZ divisible by 16;
+        the default form otherwise) (0 = tuned or default choice)."""
         out = self._scalar()
-        ws = self.workspace()
+        ws = self.magnetic_workspace()
         with torch.cuda.device(self.device):
             check_status(_lib.lib().flue_magnetic_ex(self._cdims, _ptr(self.data), self.prec, int(variant), _ptr(out),
                                               _ptr(ws), ws.numel(), _stream()), "flue_magnetic_ex")

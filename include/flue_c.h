@@ -273,10 +273,19 @@ int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_
 /* flue_magnetic with an explicit kernel variant (0 = the tuned or default choice):
  *   1..3  site-major (one thread does the three planes of its site), 1..3 waves per SIMD;
  *   4..6  plane-major (planes run one after another, sites in per-plane L2-friendly
- *         orders), 1..3 waves per SIMD.
+ *         orders), 1..3 waves per SIMD;
+ *   7     rectangle-first: each rectangle of the clover is formed once and its leaves at
+ *         the other three corners follow by conjugation (planes (x,y) and (x,z); plane
+ *         (z,y) as the default form).  It needs NX, NY, NZ divisible by 16 and a workspace
+ *         of flue_magnetic_work_bytes (about 102 bytes per site in FP32, 204 in FP64);
+ *         otherwise the call runs the default form instead.
  * Every variant is deterministic; they agree to rounding (their summation orders differ). */
 int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B, void* ws,
                      size_t ws_bytes, void* stream);
+
+/* device bytes of the workspace flue_magnetic_ex needs for every variant (flue_workspace_bytes
+ * covers variants 1..6 only); 0 for invalid arguments */
+size_t flue_magnetic_work_bytes(const int32_t dims[4], int32_t prec);
 
 /* flue_magnetic_ex without a workspace and without any allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under HIP graph
@@ -286,7 +295,8 @@ int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t
  * FLUE_EWS before any launch.  Every call on a GPU shares that GPU's array: calls that
  * overlap on one GPU (on different streams or host threads) race, so serialise them.  The
  * kernels write every partial the reduction reads, so no clearing is needed between calls.
- * Results are identical to flue_magnetic_ex with the same variant. */
+ * Results are identical to flue_magnetic_ex with the same variant; variant 7, which needs a
+ * footprint workspace, runs as the default form here. */
 int flue_magnetic_noalloc(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B,
                           void* stream);
 
@@ -363,7 +373,8 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
 /* their variant argument is 0).  The table persists through save/load.  Variants of one   */
 /* operation compute the same quantity: the flow orders and the ensemble grids give        */
 /* bit-identical results, the others agree to rounding (their summation orders differ).    */
-/*   FLUE_OP_MAGNETIC       1..6  kernel form x waves per SIMD (flue_magnetic_ex)           */
+/*   FLUE_OP_MAGNETIC       1..7  kernel form x waves per SIMD, or rectangle-first (7, timed  */
+/*                                only where it applies) (flue_magnetic_ex)                 */
 /*   FLUE_OP_B2_GRAD        1..2  per-link gather / reverse mode with slots (flue_b2_grad_ex) */
 /*   FLUE_OP_PLAQUETTE      1..6  time slices per thread 2, 3, 4, 6, 8, 12 (flue_plaquette,   */
 /*                                flue_wilson_action)                                        */

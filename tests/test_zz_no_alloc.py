@@ -1,9 +1,10 @@
 """flue_magnetic_noalloc: the magnetic B with its partial sums in the library's static
 device scratch instead of a workspace, for callers that cannot allocate.  Called through
-the C ABI and checked against flue_magnetic_ex on the same field: the kernels and the
-fixed-order reduction are the same, so the results must agree bit for bit.  Two calls in a
-row, the second on a smaller lattice, show that the shared scratch carries nothing from
-one call to the next.
+the C ABI and checked against flue_magnetic_ex on the same field given a workspace of the
+scratch's size (8 MiB): the kernels and the fixed-order reduction are the same, so the
+results must agree bit for bit.  (The rectangle-first variant 7 needs more room than that
+and runs as the default form in both.)  Two calls in a row, the second on a smaller
+lattice, show that the shared scratch carries nothing from one call to the next.
 """
 import ctypes
 
@@ -27,14 +28,21 @@ def noalloc(lib):
     return lib.flue_magnetic_noalloc
 
 
+SCRATCH = 8 << 20
+
+
 def run_both(FL, fn, g, variant):
-    from flue_lat_amd._lib import check
+    from flue_lat_amd._lib import check, lib
     out = torch.empty(1, dtype=torch.float64, device=g.data.device)
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     check(fn(g._cdims, ctypes.c_void_p(g.data.data_ptr()), g.prec, variant, ctypes.c_void_p(out.data_ptr()), st),
           "flue_magnetic_noalloc")
     got = out.item()
-    want = g.magnetic(variant=variant)
+    ws = torch.empty(SCRATCH, dtype=torch.uint8, device=g.data.device)
+    check(lib().flue_magnetic_ex(g._cdims, ctypes.c_void_p(g.data.data_ptr()), g.prec, variant,
+                                 ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()), SCRATCH, st),
+          "flue_magnetic_ex")
+    want = out.item()
     return got, want
 
 
@@ -47,9 +55,10 @@ def test_noalloc_matches_workspace_entry(FL, prec):
         print(f"[noalloc] {prec} {dims}", flush=True)
         U = O.random_field(dims, seed, 0.3)
         g = FL.GaugeField.from_complex(U, prec)
-        for variant in (0, 1, 4, 5):
+        for variant in (0, 1, 4, 5, 7):
             got, want = run_both(FL, fn, g, variant)
             assert got == want, (dims, variant, got, want)
+        assert run_both(FL, fn, g, 7)[0] == g.magnetic(variant=4 if prec == "fp64" else 5)
         tol = 1e-12 if prec == "fp64" else 1e-6
         assert abs(got - O.magnetic(U)) <= tol * O.magnetic(U)
 
@@ -58,8 +67,10 @@ def test_noalloc_benchmark_lattice(FL):
     from flue_lat_amd._lib import lib
     fn = noalloc(lib())
     g = FL.GaugeField.random((96, 48, 48, 48), 1003, 0.3).astype("fp32")
-    got, want = run_both(FL, fn, g, 0)
-    assert got == want
+    for variant in (0, 7):
+        got, want = run_both(FL, fn, g, variant)
+        assert got == want, variant
+    assert got == g.magnetic(variant=5)
     del g
     torch.cuda.empty_cache()
 
