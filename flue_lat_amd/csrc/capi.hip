@@ -94,7 +94,7 @@ bool magnetic_rect_ok(const Lat& L, int32_t prec) { return rect_fits(L) && fits_
 
 // magnetic workspace of a variant: the partials, and for variant 7 the footprints after them
 size_t magnetic_rect_partials(const Lat& L, int32_t prec) {
-    return align256(partial_bytes(2 * rect_tiles(L) + magnetic_blocks(L, prec == FLUE_F32) / 3));
+    return align256(partial_bytes(3 * rect_tiles(L)));
 }
 size_t magnetic_ws_bytes(const Lat& L, int32_t prec, int variant) {
     if (variant == kMagneticRect && magnetic_rect_ok(L, prec))

@@ -82,13 +82,11 @@ constexpr size_t kMagneticScratchBytes = (size_t)8 << 20;
 // fpbuf: the footprint buffer of variant 7 (rect_fp_bytes), unused by the others
 hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, double* partials, hipStream_t st,
                            int variant, uint32_t* npart, void* fpbuf = nullptr);
-hipError_t launch_magnetic_plane(const Lat& L, const void* U, int is_f32, int su3, int plane, double* partials,
-                                 hipStream_t st, int variant, uint32_t* npart);
 bool rect_fits(const Lat& L);
 uint32_t rect_tiles(const Lat& L);
 size_t rect_fp_bytes(const Lat& L, int is_f32);
 hipError_t launch_magnetic_rect(const Lat& L, const void* U, int is_f32, int su3, double* partials, void* fpbuf,
-                                hipStream_t st, int plane_variant, uint32_t* npart);
+                                hipStream_t st, uint32_t* npart);
 
 // gradients and flow (k_grad.hip, k_b2grad.hip); grad / Wout / Abuf are SoA in the field
 // precision, acc is FP64 (8 su(3) coordinates x 4 directions x V)

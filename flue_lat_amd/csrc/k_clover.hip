@@ -62,7 +62,7 @@ __device__ __forceinline__ int plane_nu(int pl) { return pl == 0 ? 1 : (pl == 1 
 // (src/wloops.F90:502-506, 530-543); each block writes one compensated partial sum.
 template <typename FV, int WPE, bool PLANE_MAJOR, bool C2 = sizeof(typename FV::real) == 4>
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, int pl0, double* __restrict__ partials) {
+magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, double* __restrict__ partials) {
     // C2 (FP32 always; FP64 for links declared SU(3)): link chains carry two rows and row 2
     // is rebuilt where a leaf closes.  Plain FP64 keeps full products, so it makes no
     // assumption on the links.  Consecutive quadrants share their lines where the registers
@@ -75,7 +75,7 @@ magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, int pl0, double* __rest
     constexpr bool kLA = sizeof(typename FV::real) == 8 && WPE == 1;
     // the plane comes from blockIdx.y so that it, and every buffer descriptor derived
     // from it, stays in scalar registers (a division of blockIdx.x would not)
-    const int pl = pl0 + (int)blockIdx.y;
+    const int pl = (int)blockIdx.y;
     const uint32_t blk = xcd_block(blockIdx.x, nb);
     const uint32_t o = blk * kBS + threadIdx.x;
     double acc = 0.0;
@@ -111,7 +111,7 @@ magnetic_kernel(Lat L, FV f, PlaneOrder po, uint32_t nb, int pl0, double* __rest
     }
     DD r = block_sum<kBS>(DD{acc, 0.0});
     if (threadIdx.x == 0) {
-        const uint32_t i = blockIdx.y * nb + blk;
+        const uint32_t i = (uint32_t)pl * nb + blk;
         partials[2 * i] = r.hi;
         partials[2 * i + 1] = r.lo;
     }
@@ -212,7 +212,7 @@ hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, dou
     if (variant < 1 || variant > kMagneticVariants) return hipErrorInvalidValue;
     if (variant == kMagneticRect) {
         if (!fpbuf) return hipErrorInvalidValue;
-        return launch_magnetic_rect(L, U, is_f32, su3, partials, fpbuf, st, magnetic_default_variant(is_f32), npart);
+        return launch_magnetic_rect(L, U, is_f32, su3, partials, fpbuf, st, npart);
     }
     with_view(L.V, U, is_f32, [&](auto f) {
         using F = decltype(f);
@@ -221,53 +221,25 @@ hipError_t launch_magnetic(const Lat& L, const void* U, int is_f32, int su3, dou
         const dim3 g1(nb, 1), g3(nb, 3);
         if constexpr (!F::buffered) {
             // fields beyond the buffer-resource range: one form only
-            magnetic_kernel<F, sizeof(R) == 4 ? 2 : 1, false><<<g1, kBS, 0, st>>>(L, f, po, nb, 0, partials);
+            magnetic_kernel<F, sizeof(R) == 4 ? 2 : 1, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials);
             variant = 1;
         } else if (sizeof(R) == 8 && su3 && (variant == 4 || variant == 5)) {
             if (variant == 4)
-                magnetic_kernel<F, 1, true, true><<<g3, kBS, 0, st>>>(L, f, po, nb, 0, partials);
+                magnetic_kernel<F, 1, true, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials);
             else
-                magnetic_kernel<F, 2, true, true><<<g3, kBS, 0, st>>>(L, f, po, nb, 0, partials);
+                magnetic_kernel<F, 2, true, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials);
         } else {
             switch (variant) {
-                case 1: magnetic_kernel<F, 1, false><<<g1, kBS, 0, st>>>(L, f, po, nb, 0, partials); break;
-                case 2: magnetic_kernel<F, 2, false><<<g1, kBS, 0, st>>>(L, f, po, nb, 0, partials); break;
-                case 3: magnetic_kernel<F, 3, false><<<g1, kBS, 0, st>>>(L, f, po, nb, 0, partials); break;
-                case 4: magnetic_kernel<F, 1, true><<<g3, kBS, 0, st>>>(L, f, po, nb, 0, partials); break;
-                case 5: magnetic_kernel<F, 2, true><<<g3, kBS, 0, st>>>(L, f, po, nb, 0, partials); break;
-                default: magnetic_kernel<F, 3, true><<<g3, kBS, 0, st>>>(L, f, po, nb, 0, partials); break;
+                case 1: magnetic_kernel<F, 1, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                case 2: magnetic_kernel<F, 2, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                case 3: magnetic_kernel<F, 3, false><<<g1, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                case 4: magnetic_kernel<F, 1, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                case 5: magnetic_kernel<F, 2, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials); break;
+                default: magnetic_kernel<F, 3, true><<<g3, kBS, 0, st>>>(L, f, po, nb, partials); break;
             }
         }
     });
     if (npart) *npart = variant >= 4 ? 3 * nb : nb;
-    return hipGetLastError();
-}
-
-// one magnetic plane (0 (y,x), 1 (x,z), 2 (z,y)) in the plane-major form of `variant` (4..6),
-// one compensated partial per block; returns the number of partials through *npart
-hipError_t launch_magnetic_plane(const Lat& L, const void* U, int is_f32, int su3, int plane, double* partials,
-                                 hipStream_t st, int variant, uint32_t* npart) {
-    const uint32_t nb = magnetic_blocks(L, is_f32) / 3;
-    if (plane < 0 || plane > 2 || variant < 4 || variant > 6 || !fits_buffer(L.V, is_f32)) return hipErrorInvalidValue;
-    with_view(L.V, U, is_f32, [&](auto f) {
-        using F = decltype(f);
-        using R = typename F::real;
-        if constexpr (F::buffered) {
-            const PlaneOrder po = plane_order(L, sizeof(R));
-            const dim3 g(nb, 1);
-            if (sizeof(R) == 8 && su3 && variant == 4)
-                magnetic_kernel<F, 1, true, true><<<g, kBS, 0, st>>>(L, f, po, nb, plane, partials);
-            else if (sizeof(R) == 8 && su3 && variant == 5)
-                magnetic_kernel<F, 2, true, true><<<g, kBS, 0, st>>>(L, f, po, nb, plane, partials);
-            else if (variant == 4)
-                magnetic_kernel<F, 1, true><<<g, kBS, 0, st>>>(L, f, po, nb, plane, partials);
-            else if (variant == 5)
-                magnetic_kernel<F, 2, true><<<g, kBS, 0, st>>>(L, f, po, nb, plane, partials);
-            else
-                magnetic_kernel<F, 3, true><<<g, kBS, 0, st>>>(L, f, po, nb, plane, partials);
-        }
-    });
-    if (npart) *npart = nb;
     return hipGetLastError();
 }
 

@@ -9,7 +9,9 @@
 // cell's sum is formed in a fixed order.  The footprints go to memory; the gather adds to
 // each site of a tile the overlapping cells of the tiles before it along x, along the second
 // direction and diagonally (in that fixed order), and sums |F|^2.  Plane (z, y) has no x
-// direction to coalesce along and runs the plane-major kernel of the other variants.
+// direction to coalesce along: its z and y links are first copied with z fastest (an LDS-tiled
+// transpose), which makes it plane (x', y') of the lattice with extents (NT, NZ, NY, NX), and
+// the same two kernels run there.
 #include "flue_rect.hpp"
 #include "flue_launch.hpp"
 
@@ -174,44 +176,94 @@ __global__ void __launch_bounds__(kRW) rect_gather_kernel(Lat L, const R* __rest
     }
 }
 
+// U_z and U_y of the field (lattice L) into P, z fastest: P holds two direction blocks of
+// 9 complex planes, [k][e][s'] with s' = z + NZ (y + NY (x + NX t)), k = 0 for U_z, 1 for
+// U_y.  One block per 16 x 16 (x, z) tile at fixed (y, t), through a padded LDS tile so that
+// both the reads (along x) and the writes (along z) coalesce.
+template <typename R>
+__global__ void __launch_bounds__(256) zy_relayout_kernel(Lat L, const R* __restrict__ U, R* __restrict__ P) {
+    using P2 = typename Pair2<R>::type;
+    __shared__ P2 tile[16][17];
+    const uint32_t NX = (uint32_t)L.n[1], NY = (uint32_t)L.n[2], NZ = (uint32_t)L.n[3];
+    uint32_t q = blockIdx.x;
+    const uint32_t bx = q % (NX / 16);
+    q /= NX / 16;
+    const uint32_t bz = q % (NZ / 16);
+    q /= NZ / 16;
+    const uint32_t y = q % NY, t = q / NY;
+    const uint32_t i = threadIdx.x >> 4, j = threadIdx.x & 15;
+    const uint32_t s = bx * 16 + j + NX * (y + NY * (bz * 16 + i + NZ * t));
+    const uint32_t s2 = bz * 16 + j + NZ * (y + NY * (bx * 16 + i + NX * t));
+    const P2* src = reinterpret_cast<const P2*>(U);
+    P2* dst = reinterpret_cast<P2*>(P);
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const int mu = k == 0 ? 3 : 2;
+#pragma unroll 1
+        for (int e = 0; e < 9; ++e) {
+            tile[i][j] = src[(size_t)(mu * 9 + e) * L.V + s];
+            __syncthreads();
+            dst[(size_t)(k * 9 + e) * L.V + s2] = tile[j][i];
+            __syncthreads();
+        }
+    }
+}
+
+// lattice whose x, y, z are the z, y, x of L: its plane (x, y) is L's plane (z, y)
+Lat zy_lattice(const Lat& L) {
+    const int32_t d[4] = {L.n[0], L.n[3], L.n[2], L.n[1]};
+    return make_lat(d);
+}
+
 }  // namespace
 
 bool rect_fits(const Lat& L) { return L.n[1] % kRT == 0 && L.n[2] % kRT == 0 && L.n[3] % kRT == 0; }
 
 uint32_t rect_tiles(const Lat& L) { return L.V / (kRT * kRT); }
 
+// the three planes' footprints, then the z-fastest copy of U_z and U_y
 size_t rect_fp_bytes(const Lat& L, int is_f32) {
-    return (size_t)2 * rect_tiles(L) * 9 * kFC * (is_f32 ? 4 : 8);
+    const size_t sz = is_f32 ? 4 : 8;
+    return (size_t)3 * rect_tiles(L) * 9 * kFC * sz + (size_t)36 * L.V * sz;
 }
 
-// partials: 2 * rect_tiles for planes (x, y) and (x, z), then the plane-major kernel's for (z, y)
+// partials: rect_tiles for each of the planes (x, y), (x, z), (z, y)
 hipError_t launch_magnetic_rect(const Lat& L, const void* U, int is_f32, int su3, double* partials, void* fpbuf,
-                                hipStream_t st, int plane_variant, uint32_t* npart) {
+                                hipStream_t st, uint32_t* npart) {
     if (!rect_fits(L) || !fits_buffer(L.V, is_f32)) return hipErrorInvalidValue;
     const uint32_t nt = rect_tiles(L);
+    const Lat Lzy = zy_lattice(L);
     with_view(L.V, U, is_f32, [&](auto f) {
         using F = decltype(f);
         using R = typename F::real;
         if constexpr (F::buffered) {
-            const dim3 g(nt, 2);
+            R* fp = (R*)fpbuf;
+            R* fpzy = fp + (size_t)2 * nt * 9 * kFC;
+            R* P = fp + (size_t)3 * nt * 9 * kFC;
+            zy_relayout_kernel<R><<<L.V / 256, 256, 0, st>>>(L, f.p, P);
+            // the copy read as a field of Lzy whose directions 1 and 2 are its two blocks (the
+            // view's base is one direction block before P; directions 0 and 3 are never read)
+            const F fzy{(const R*)((uintptr_t)P - (uintptr_t)18 * L.V * sizeof(R)), L.V};
+            const dim3 g2(nt, 2), g1(nt, 1);
             // FP32 and declared-SU(3) FP64 links: two-row chains; plain FP64: full products
             if constexpr (sizeof(R) == 4) {
-                rect_kernel<F, true, 2><<<g, kRW, 0, st>>>(L, f, nt, (R*)fpbuf);
+                rect_kernel<F, true, 2><<<g2, kRW, 0, st>>>(L, f, nt, fp);
+                rect_kernel<F, true, 2><<<g1, kRW, 0, st>>>(Lzy, fzy, nt, fpzy);
             } else {
-                if (su3)
-                    rect_kernel<F, true, 1><<<g, kRW, 0, st>>>(L, f, nt, (R*)fpbuf);
-                else
-                    rect_kernel<F, false, 1><<<g, kRW, 0, st>>>(L, f, nt, (R*)fpbuf);
+                if (su3) {
+                    rect_kernel<F, true, 1><<<g2, kRW, 0, st>>>(L, f, nt, fp);
+                    rect_kernel<F, true, 1><<<g1, kRW, 0, st>>>(Lzy, fzy, nt, fpzy);
+                } else {
+                    rect_kernel<F, false, 1><<<g2, kRW, 0, st>>>(L, f, nt, fp);
+                    rect_kernel<F, false, 1><<<g1, kRW, 0, st>>>(Lzy, fzy, nt, fpzy);
+                }
             }
-            rect_gather_kernel<R><<<g, kRW, 0, st>>>(L, (const R*)fpbuf, nt, partials);
+            rect_gather_kernel<R><<<g2, kRW, 0, st>>>(L, fp, nt, partials);
+            rect_gather_kernel<R><<<g1, kRW, 0, st>>>(Lzy, fpzy, nt, partials + (size_t)2 * 2 * nt);
         }
     });
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    uint32_t np2 = 0;
-    e = launch_magnetic_plane(L, U, is_f32, su3, 2, partials + 2 * (size_t)2 * nt, st, plane_variant, &np2);
-    if (npart) *npart = 2 * nt + np2;
-    return e;
+    if (npart) *npart = 3 * nt;
+    return hipGetLastError();
 }
 
 }  // namespace flue

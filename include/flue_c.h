@@ -275,9 +275,9 @@ int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_
  *   4..6  plane-major (planes run one after another, sites in per-plane L2-friendly
  *         orders), 1..3 waves per SIMD;
  *   7     rectangle-first: each rectangle of the clover is formed once and its leaves at
- *         the other three corners follow by conjugation (planes (x,y) and (x,z); plane
- *         (z,y) as the default form).  It needs NX, NY, NZ divisible by 16 and a workspace
- *         of flue_magnetic_work_bytes (about 102 bytes per site in FP32, 204 in FP64);
+ *         the other three corners follow by conjugation (plane (z,y) from a z-fastest copy
+ *         of its links).  It needs NX, NY, NZ divisible by 16 and a workspace of
+ *         flue_magnetic_work_bytes (about 300 bytes per site in FP32, 600 in FP64);
  *         otherwise the call runs the default form instead.
  * Every variant is deterministic; they agree to rounding (their summation orders differ). */
 int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B, void* ws,
