@@ -164,8 +164,10 @@ __device__ __forceinline__ AH<R> rect_close(R w, const Mat<R>& CA, const Mat<R>&
 }
 
 // The seven rectangles with lower-left corner x in quadrant q = (+d1, +d2).  emit(da, db, p)
-// adds the AH parts p to the site x + da d1 + db d2; the calls come in a fixed order.
-template <bool C2, typename FV, typename Emit>
+// adds the AH parts p to the site x + da d1 + db d2; the calls come in a fixed order.  With
+// LA each group of products also loads the link of the next group's first product (P), so
+// that product starts on a link in registers (for forms at one wave per SIMD).
+template <bool C2, bool LA, typename FV, typename Emit>
 __device__ __forceinline__ void rect_base(const Quad<FV>& q, Emit&& emit) {
     using R = typename FV::real;
     const R w11 = R(kK1), w22 = R(kK2), w33 = R(kK5), w12 = R(0.5 * kK3), w13 = R(0.5 * kK4);
@@ -175,46 +177,53 @@ __device__ __forceinline__ void rect_base(const Quad<FV>& q, Emit&& emit) {
     ah_zero(gb1);
     ah_zero(gb2);
     ah_zero(gb3);
+    Mat<R> P;
     // row a = 1
     const Mat<R> L1 = first_link<1, 1, 0, 0>(q);   // Line(x,d1,1)
     Mat<R> CA = ext<1, 2, 1, 0, C2>(q, L1);            // CA(1,1)
     const Mat<R> Ld1 = first_link<1, 2, 0, 0>(q);  // Line(x,d2,1)
     Mat<R> CB1 = ext<1, 1, 0, 1, C2>(q, Ld1);          // CB(1,1)
+    if constexpr (LA) P = ext_link<1, 2, 1, 1>(q);
     emit(1, 1, rect_close<C2>(w11, CA, CB1, h0, ga, gb1));
     FLUE_SCHED_FENCE();
-    CA = ext<1, 2, 1, 1, C2>(q, CA);                   // CA(1,2)
+    CA = LA ? ext_by<1, C2>(CA, P) : ext<1, 2, 1, 1, C2>(q, CA);  // CA(1,2)
     const Mat<R> Ld2 = ext<1, 2, 0, 1, C2>(q, Ld1);    // Line(x,d2,2)
     Mat<R> CB2 = ext<1, 1, 0, 2, C2>(q, Ld2);          // CB(2,1)
+    if constexpr (LA) P = ext_link<1, 2, 1, 2>(q);
     emit(1, 2, rect_close<C2>(w12, CA, CB2, h0, ga, gb2));
     FLUE_SCHED_FENCE();
-    CA = ext<1, 2, 1, 2, C2>(q, CA);                   // CA(1,3)
+    CA = LA ? ext_by<1, C2>(CA, P) : ext<1, 2, 1, 2, C2>(q, CA);  // CA(1,3)
     const Mat<R> Ld3 = ext<1, 2, 0, 2, C2>(q, Ld2);    // Line(x,d2,3)
     Mat<R> CB3 = ext<1, 1, 0, 3, C2>(q, Ld3);          // CB(3,1)
+    if constexpr (LA) P = ext_link<1, 1, 1, 0>(q);
     emit(1, 3, rect_close<C2>(w13, CA, CB3, h0, ga, gb3));
     emit(1, 0, ah_conj(ga, L1));
     ah_zero(ga);
     FLUE_SCHED_FENCE();
     // row a = 2
-    Mat<R> L = ext<1, 1, 1, 0, C2>(q, L1);             // Line(x,d1,2)
+    Mat<R> L = LA ? ext_by<1, C2>(L1, P) : ext<1, 1, 1, 0, C2>(q, L1);  // Line(x,d1,2)
     CA = ext<1, 2, 2, 0, C2>(q, L);                    // CA(2,1)
     CB1 = ext<1, 1, 1, 1, C2>(q, CB1);                 // CB(1,2)
+    if constexpr (LA) P = ext_link<1, 2, 2, 1>(q);
     emit(2, 1, rect_close<C2>(w12, CA, CB1, h0, ga, gb1));
     FLUE_SCHED_FENCE();
-    CA = ext<1, 2, 2, 1, C2>(q, CA);                   // CA(2,2)
+    CA = LA ? ext_by<1, C2>(CA, P) : ext<1, 2, 2, 1, C2>(q, CA);  // CA(2,2)
     CB2 = ext<1, 1, 1, 2, C2>(q, CB2);                 // CB(2,2)
+    if constexpr (LA) P = ext_link<1, 1, 2, 0>(q);
     emit(2, 2, rect_close<C2>(w22, CA, CB2, h0, ga, gb2));
     emit(2, 0, ah_conj(ga, full_rows<C2>(L)));
     ah_zero(ga);
     emit(0, 2, ah_conj(gb2, full_rows<C2>(Ld2)));  // b = 2: (1,2), (2,2)
     FLUE_SCHED_FENCE();
     // row a = 3
-    L = ext<1, 1, 2, 0, C2>(q, L);                     // Line(x,d1,3)
+    L = LA ? ext_by<1, C2>(L, P) : ext<1, 1, 2, 0, C2>(q, L);  // Line(x,d1,3)
     CA = ext<1, 2, 3, 0, C2>(q, L);                    // CA(3,1)
     CB1 = ext<1, 1, 2, 1, C2>(q, CB1);                 // CB(1,3)
+    if constexpr (LA) P = ext_link<1, 2, 3, 1>(q);
     emit(3, 1, rect_close<C2>(w13, CA, CB1, h0, ga, gb1));
     emit(0, 1, ah_conj(gb1, Ld1));                 // b = 1: (1,1), (2,1), (3,1)
     FLUE_SCHED_FENCE();
-    CA = ext<1, 2, 3, 1, C2>(q, CA);                   // CA(3,2)
+    CA = LA ? ext_by<1, C2>(CA, P) : ext<1, 2, 3, 1, C2>(q, CA);  // CA(3,2)
     CA = ext<1, 2, 3, 2, C2>(q, CA);                   // CA(3,3)
     CB3 = ext<1, 1, 1, 3, C2>(q, CB3);                 // CB(3,2)
     CB3 = ext<1, 1, 2, 3, C2>(q, CB3);                 // CB(3,3)

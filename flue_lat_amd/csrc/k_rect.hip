@@ -93,6 +93,8 @@ __device__ __forceinline__ void fp_add(R* fp, int cell, const AH<R>& p) {
 template <typename R>
 constexpr int kCellReals = sizeof(R) == 4 ? 12 : 9;
 
+// WPE == 1 (FP64) takes the chains with a one-link lookahead (rect_base); at two waves per
+// SIMD (FP32) the other wave covers the load latency
 template <typename FV, bool C2, int WPE>
 __global__ void __launch_bounds__(kRW) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 rect_kernel(Lat L, FV f, uint32_t ntiles, typename FV::real* __restrict__ fpout) {
@@ -113,7 +115,7 @@ rect_kernel(Lat L, FV f, uint32_t ntiles, typename FV::real* __restrict__ fpout)
         const int cy = rp == 0 ? c2 : (int)ti.cs, cz = rp == 0 ? (int)ti.cs : c2;
         const uint32_t s = (uint32_t)cx + L.st[2] * (uint32_t)cy + L.st[3] * (uint32_t)cz + L.st[0] * ti.ct;
         const PlaneGeom<FV> g = plane_geom(L, f, s, cx, c2, 1, d2);
-        rect_base<C2>(make_quad<1, 1, true>(g), [&](int da, int db, const AH<R>& p) {
+        rect_base<C2, WPE == 1>(make_quad<1, 1, true>(g), [&](int da, int db, const AH<R>& p) {
             fp_add(fp, (ly + db) * kFP + lx + da, p);
         });
     }
@@ -131,8 +133,11 @@ rect_kernel(Lat L, FV f, uint32_t ntiles, typename FV::real* __restrict__ fpout)
 
 // per tile: the 256 sites' clover AH parts from the four footprints that cover them, then
 // sum |F|^2 into one compensated partial per tile
+// gather block: one wavefront, four sites per thread (measured faster than a thread per site)
+constexpr int kGB = 64;
+
 template <typename R>
-__global__ void __launch_bounds__(kRW) rect_gather_kernel(Lat L, const R* __restrict__ fpin, uint32_t ntiles,
+__global__ void __launch_bounds__(kGB) rect_gather_kernel(Lat L, const R* __restrict__ fpin, uint32_t ntiles,
                                                          double* __restrict__ partials) {
     const int rp = (int)blockIdx.y;
     const uint32_t tile = xcd_block(blockIdx.x, ntiles);
@@ -146,9 +151,9 @@ __global__ void __launch_bounds__(kRW) rect_gather_kernel(Lat L, const R* __rest
     const R* diag = base + (size_t)tile_of(L, rp, txl, t2b, ti.cs, ti.ct) * (9 * kFC);
     const int cx = (int)threadIdx.x & 15;
     double acc = 0.0;
-#pragma unroll 1
-    for (int r = 0; r < kRT / 4; ++r) {
-        const int cy = ((int)threadIdx.x >> 4) + 4 * r;
+#pragma unroll
+    for (int r = 0; r < kRT * kRT / kGB; ++r) {
+        const int cy = ((int)threadIdx.x >> 4) + r * (kGB / kRT);
         const int c = cy * kFP + cx;
         R v[9];
 #pragma unroll
@@ -168,7 +173,7 @@ __global__ void __launch_bounds__(kRW) rect_gather_kernel(Lat L, const R* __rest
         }
         acc += f_norm2(h);
     }
-    const DD s = block_sum<kRW>(DD{acc, 0.0});
+    const DD s = block_sum<kGB>(DD{acc, 0.0});
     if (threadIdx.x == 0) {
         const uint32_t i = (uint32_t)rp * ntiles + tile;
         partials[2 * i] = s.hi;
@@ -258,8 +263,8 @@ hipError_t launch_magnetic_rect(const Lat& L, const void* U, int is_f32, int su3
                     rect_kernel<F, false, 1><<<g1, kRW, 0, st>>>(Lzy, fzy, nt, fpzy);
                 }
             }
-            rect_gather_kernel<R><<<g2, kRW, 0, st>>>(L, fp, nt, partials);
-            rect_gather_kernel<R><<<g1, kRW, 0, st>>>(Lzy, fpzy, nt, partials + (size_t)2 * 2 * nt);
+            rect_gather_kernel<R><<<g2, kGB, 0, st>>>(L, fp, nt, partials);
+            rect_gather_kernel<R><<<g1, kGB, 0, st>>>(Lzy, fpzy, nt, partials + (size_t)2 * 2 * nt);
         }
     });
     if (npart) *npart = 3 * nt;
