@@ -1,16 +1,18 @@
 """Benchmark: Gsites/s of the 5-loop F_munu + B^2 (magnetic) measurement on a 48^3x96
 lattice, FP32 links with FP64 sums versus the plain FP64 path, on one MI355X.
 
-A step is one full measurement of B over the lattice (the field-strength kernel for
-the three spatial planes fused with the -ReTr F^2 reduction, plus the fixed-order
-second pass), with the links already resident in HBM in the SoA layout.  The field is
+A step is one full measurement of B over the lattice in the variant the autotuner picks
+on the benchmark field (the rectangle-first form at 48^3x96: the z-fastest copy of two
+link directions, the tile kernels for the three spatial planes, their gathers with the
+-ReTr F^2 sums, and the fixed-order second pass), with the links already resident in
+HBM in the SoA layout.  The field is
 synthetic: the seeded SU(3) generator with eps = 0.3, generated in FP64 on the GPU
 and rounded to FP32 for the FP32 path.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
 Prints one JSON line.  Beside the headline it carries the FP64 line, the roofline of the
-magnetic kernel, event-timed rows of the other operations, the ensemble-flow workload
+magnetic measurement, event-timed rows of the other operations, the ensemble-flow workload
 (32 configurations of 16^3 x Nt, 100 RK3 steps in one hipGraph) and, unless --no-cpu,
 the CPU oracle on the same 48^3x96 field: its time is the CPU baseline and its B,
 plaquette and Polyakov sums give each observable's error (rel_err_vs_oracle).
@@ -171,8 +173,8 @@ def ensemble_leg(FL, torch):
 
 
 def hbm_traffic():
-    """HBM bytes per launch of the FP32 magnetic kernel, from the FETCH_SIZE/WRITE_SIZE
-    counter summary kept with the profiles (None when that summary is absent)."""
+    """HBM-side bytes of one FP32 B measurement (all its dispatches), from the FETCH_SIZE /
+    WRITE_SIZE counter summary kept with the profiles (None when that summary is absent)."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     m = re.search(r"^hbm_bytes_per_launch_fp32\s*=\s*(\d+)", open(TRAFFIC_FILE).read(), flags=re.M)
