@@ -97,11 +97,15 @@ constexpr int kCellReals = sizeof(R) == 4 ? 12 : 9;
 // SIMD (FP32) the other wave covers the load latency
 template <typename FV, bool C2, int WPE>
 __global__ void __launch_bounds__(kRW) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-rect_kernel(Lat L, FV f, uint32_t ntiles, typename FV::real* __restrict__ fpout) {
+rect_kernel(Lat L0, FV f0, Lat Lzy, FV fzy, uint32_t ntiles, typename FV::real* __restrict__ fpout) {
     using R = typename FV::real;
     constexpr int CR = kCellReals<R>;
     __shared__ __align__(16) R fp[CR * kFC];
-    const int rp = (int)blockIdx.y;
+    // blockIdx.y: 0 plane (x,y), 1 plane (x,z), 2 plane (z,y) as plane (x,y) of the copy
+    const bool zy = blockIdx.y == 2;
+    const Lat L = zy ? Lzy : L0;
+    const FV f = zy ? fzy : f0;
+    const int rp = zy ? 0 : (int)blockIdx.y;
     const int d2 = rect_d2(rp);
     const uint32_t tile = xcd_block(blockIdx.x, ntiles);
     const TileIdx ti = tile_idx(L, rp, tile);
@@ -121,7 +125,7 @@ rect_kernel(Lat L, FV f, uint32_t ntiles, typename FV::real* __restrict__ fpout)
     }
     __syncthreads();
     // to memory as [e * kFC + c] in both precisions
-    R* out = fpout + ((size_t)rp * ntiles + tile) * (9 * kFC);
+    R* out = fpout + ((size_t)blockIdx.y * ntiles + tile) * (9 * kFC);
     if constexpr (CR == 12) {
         for (int c = threadIdx.x; c < kFC; c += kRW)
 #pragma unroll
@@ -137,14 +141,16 @@ rect_kernel(Lat L, FV f, uint32_t ntiles, typename FV::real* __restrict__ fpout)
 constexpr int kGB = 64;
 
 template <typename R>
-__global__ void __launch_bounds__(kGB) rect_gather_kernel(Lat L, const R* __restrict__ fpin, uint32_t ntiles,
-                                                         double* __restrict__ partials) {
-    const int rp = (int)blockIdx.y;
+__global__ void __launch_bounds__(kGB) rect_gather_kernel(Lat L0, Lat Lzy, const R* __restrict__ fpin,
+                                                         uint32_t ntiles, double* __restrict__ partials) {
+    const bool zy = blockIdx.y == 2;
+    const Lat L = zy ? Lzy : L0;
+    const int rp = zy ? 0 : (int)blockIdx.y;
     const uint32_t tile = xcd_block(blockIdx.x, ntiles);
     const TileIdx ti = tile_idx(L, rp, tile);
     const uint32_t ntx = (uint32_t)L.n[1] / kRT, nt2 = (uint32_t)L.n[rect_d2(rp)] / kRT;
     const uint32_t txl = (ti.tx + ntx - 1) % ntx, t2b = (ti.t2 + nt2 - 1) % nt2;
-    const R* base = fpin + (size_t)rp * ntiles * (9 * kFC);
+    const R* base = fpin + (size_t)blockIdx.y * ntiles * (9 * kFC);
     const R* own = base + (size_t)tile * (9 * kFC);
     const R* left = base + (size_t)tile_of(L, rp, txl, ti.t2, ti.cs, ti.ct) * (9 * kFC);
     const R* below = base + (size_t)tile_of(L, rp, ti.tx, t2b, ti.cs, ti.ct) * (9 * kFC);
@@ -175,7 +181,7 @@ __global__ void __launch_bounds__(kGB) rect_gather_kernel(Lat L, const R* __rest
     }
     const DD s = block_sum<kGB>(DD{acc, 0.0});
     if (threadIdx.x == 0) {
-        const uint32_t i = (uint32_t)rp * ntiles + tile;
+        const uint32_t i = blockIdx.y * ntiles + tile;
         partials[2 * i] = s.hi;
         partials[2 * i + 1] = s.lo;
     }
@@ -243,28 +249,22 @@ hipError_t launch_magnetic_rect(const Lat& L, const void* U, int is_f32, int su3
         using R = typename F::real;
         if constexpr (F::buffered) {
             R* fp = (R*)fpbuf;
-            R* fpzy = fp + (size_t)2 * nt * 9 * kFC;
             R* P = fp + (size_t)3 * nt * 9 * kFC;
             zy_relayout_kernel<R><<<L.V / 256, 256, 0, st>>>(L, f.p, P);
             // the copy read as a field of Lzy whose directions 1 and 2 are its two blocks (the
             // view's base is one direction block before P; directions 0 and 3 are never read)
             const F fzy{(const R*)((uintptr_t)P - (uintptr_t)18 * L.V * sizeof(R)), L.V};
-            const dim3 g2(nt, 2), g1(nt, 1);
+            const dim3 g(nt, 3);
             // FP32 and declared-SU(3) FP64 links: two-row chains; plain FP64: full products
             if constexpr (sizeof(R) == 4) {
-                rect_kernel<F, true, 2><<<g2, kRW, 0, st>>>(L, f, nt, fp);
-                rect_kernel<F, true, 2><<<g1, kRW, 0, st>>>(Lzy, fzy, nt, fpzy);
+                rect_kernel<F, true, 2><<<g, kRW, 0, st>>>(L, f, Lzy, fzy, nt, fp);
             } else {
-                if (su3) {
-                    rect_kernel<F, true, 1><<<g2, kRW, 0, st>>>(L, f, nt, fp);
-                    rect_kernel<F, true, 1><<<g1, kRW, 0, st>>>(Lzy, fzy, nt, fpzy);
-                } else {
-                    rect_kernel<F, false, 1><<<g2, kRW, 0, st>>>(L, f, nt, fp);
-                    rect_kernel<F, false, 1><<<g1, kRW, 0, st>>>(Lzy, fzy, nt, fpzy);
-                }
+                if (su3)
+                    rect_kernel<F, true, 1><<<g, kRW, 0, st>>>(L, f, Lzy, fzy, nt, fp);
+                else
+                    rect_kernel<F, false, 1><<<g, kRW, 0, st>>>(L, f, Lzy, fzy, nt, fp);
             }
-            rect_gather_kernel<R><<<g2, kGB, 0, st>>>(L, fp, nt, partials);
-            rect_gather_kernel<R><<<g1, kGB, 0, st>>>(Lzy, fpzy, nt, partials + (size_t)2 * 2 * nt);
+            rect_gather_kernel<R><<<g, kGB, 0, st>>>(L, Lzy, fp, nt, partials);
         }
     });
     if (npart) *npart = 3 * nt;
