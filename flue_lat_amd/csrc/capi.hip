@@ -120,15 +120,27 @@ size_t b2_ah_bytes(const Lat& L, int32_t prec) {
     return ((size_t)27 * L.V * (prec == FLUE_F32 ? 4 : 8) + 255) & ~(size_t)255;
 }
 
-// the B^2 adjoint after the SU(3) check: clover sums into the start of `work`, then variant
-// 1 (per-link gather) or 2 (reverse mode with the slot buffer after the clover sums)
+// the region after the clover sums: the reverse form's slot buffer, and before it, where the
+// rectangle-first magnetic form applies, that form's footprints and link copy
+size_t b2_region_bytes(const Lat& L, int32_t prec) {
+    const size_t slots = b2_slot_bytes(L, prec == FLUE_F32);
+    return magnetic_rect_ok(L, prec) ? std::max(slots, rect_fp_bytes(L, prec == FLUE_F32)) : slots;
+}
+
+// the B^2 adjoint after the SU(3) check: clover sums into the start of `work` (rectangle-first
+// where it applies), then variant 1 (per-link gather) or 2 (reverse mode with the slot buffer
+// after the clover sums)
 int b2_grad_run(const Lat& L, const void* U, int32_t prec, int variant, void* work, void* grad, hipStream_t st) {
     const int f32 = prec == FLUE_F32;
-    FLUE_TRY(launch_magnetic_ah(L, U, f32, work, st));
+    void* region = (char*)work + b2_ah_bytes(L, prec);
+    if (magnetic_rect_ok(L, prec))
+        FLUE_TRY(launch_magnetic_rect(L, U, f32, 0, nullptr, region, st, nullptr, work));
+    else
+        FLUE_TRY(launch_magnetic_ah(L, U, f32, work, st));
     if (variant == 1)
         FLUE_TRY(launch_b2_grad(L, U, f32, work, grad, st));
     else
-        FLUE_TRY(launch_b2_grad_rev(L, U, f32, work, (char*)work + b2_ah_bytes(L, prec), grad, st));
+        FLUE_TRY(launch_b2_grad_rev(L, U, f32, work, region, grad, st));
     return FLUE_OK;
 }
 
@@ -745,7 +757,7 @@ size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec) {
     const Lat L = make_lat(dims);
     // the per-site anti-Hermitian clover sums, then the slot buffer of the reverse form (one
     // slab of time slices); the SU(3) check uses the start of the buffer first
-    return std::max(b2_ah_bytes(L, prec) + b2_slot_bytes(L, prec == FLUE_F32), check_bytes(L));
+    return std::max(b2_ah_bytes(L, prec) + b2_region_bytes(L, prec), check_bytes(L));
 }
 
 size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec) {

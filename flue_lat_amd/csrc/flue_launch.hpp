@@ -86,7 +86,7 @@ bool rect_fits(const Lat& L);
 uint32_t rect_tiles(const Lat& L);
 size_t rect_fp_bytes(const Lat& L, int is_f32);
 hipError_t launch_magnetic_rect(const Lat& L, const void* U, int is_f32, int su3, double* partials, void* fpbuf,
-                                hipStream_t st, uint32_t* npart);
+                                hipStream_t st, uint32_t* npart, void* Abuf = nullptr);
 
 // gradients and flow (k_grad.hip, k_b2grad.hip); grad / Wout / Abuf are SoA in the field
 // precision, acc is FP64 (8 su(3) coordinates x 4 directions x V)

@@ -140,9 +140,38 @@ rect_kernel(Lat L0, FV f0, Lat Lzy, FV fzy, uint32_t ntiles, typename FV::real* 
 // gather block: one wavefront, four sites per thread (measured faster than a thread per site)
 constexpr int kGB = 64;
 
+// the clover AH parts of tile cell (cx, cy): its own footprint's cell, then the overlapping
+// cells of the tiles before it along x, along d2 and diagonally, in that order
 template <typename R>
+__device__ __forceinline__ AH<R> cell_sum(const R* own, const R* left, const R* below, const R* diag, int cx,
+                                          int cy) {
+    const int c = cy * kFP + cx;
+    R v[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        R x = own[e * kFC + c];
+        if (cx < 3) x += left[e * kFC + c + kRT];
+        if (cy < 3) x += below[e * kFC + c + kRT * kFP];
+        if (cx < 3 && cy < 3) x += diag[e * kFC + c + kRT * kFP + kRT];
+        v[e] = x;
+    }
+    AH<R> h;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        h.d[k] = v[k];
+        h.or_[k] = v[3 + k];
+        h.oi[k] = v[6 + k];
+    }
+    return h;
+}
+
+// STORE = false: sum |F|^2 over the tile into one compensated partial per tile.  STORE = true:
+// write each site's AH parts into Abuf in the B^2 adjoint's layout (9 planes of V values per
+// magnetic plane (y,x), (x,z), (z,y): d0 d1 d2, o01r o02r o12r, o01i o02i o12i).
+template <typename R, bool STORE>
 __global__ void __launch_bounds__(kGB) rect_gather_kernel(Lat L0, Lat Lzy, const R* __restrict__ fpin,
-                                                         uint32_t ntiles, double* __restrict__ partials) {
+                                                         uint32_t ntiles, double* __restrict__ partials,
+                                                         R* __restrict__ Abuf) {
     const bool zy = blockIdx.y == 2;
     const Lat L = zy ? Lzy : L0;
     const int rp = zy ? 0 : (int)blockIdx.y;
@@ -160,30 +189,37 @@ __global__ void __launch_bounds__(kGB) rect_gather_kernel(Lat L0, Lat Lzy, const
 #pragma unroll
     for (int r = 0; r < kRT * kRT / kGB; ++r) {
         const int cy = ((int)threadIdx.x >> 4) + r * (kGB / kRT);
-        const int c = cy * kFP + cx;
-        R v[9];
+        if constexpr (STORE) {
+            const AH<R> h = cell_sum(own, left, below, diag, cx, cy);
+            // the site in the field's own lattice: for plane 2, the copy's x', y', z' are z, y, x
+            const uint32_t c2 = ti.t2 * kRT + (uint32_t)cy, c1 = ti.tx * kRT + (uint32_t)cx;
+            uint32_t s;
+            if (zy)
+                s = ti.cs + L0.st[2] * c2 + L0.st[3] * c1 + L0.st[0] * ti.ct;
+            else if (rp == 0)
+                s = c1 + L0.st[2] * c2 + L0.st[3] * ti.cs + L0.st[0] * ti.ct;
+            else
+                s = c1 + L0.st[2] * ti.cs + L0.st[3] * c2 + L0.st[0] * ti.ct;
+            // the adjoint's planes are (y,x), (x,z), (z,y): the first is the tiles' (x,y) reversed
+            const R sg = blockIdx.y == 0 ? R(-1) : R(1);
+            R* o = Abuf + (size_t)blockIdx.y * 9 * L0.V + s;
 #pragma unroll
-        for (int e = 0; e < 9; ++e) {
-            R x = own[e * kFC + c];
-            if (cx < 3) x += left[e * kFC + c + kRT];
-            if (cy < 3) x += below[e * kFC + c + kRT * kFP];
-            if (cx < 3 && cy < 3) x += diag[e * kFC + c + kRT * kFP + kRT];
-            v[e] = x;
+            for (int k = 0; k < 3; ++k) {
+                o[(size_t)k * L0.V] = sg * h.d[k];
+                o[(size_t)(3 + k) * L0.V] = sg * h.or_[k];
+                o[(size_t)(6 + k) * L0.V] = sg * h.oi[k];
+            }
+        } else {
+            acc += f_norm2(cell_sum(own, left, below, diag, cx, cy));
         }
-        AH<R> h;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            h.d[k] = v[k];
-            h.or_[k] = v[3 + k];
-            h.oi[k] = v[6 + k];
-        }
-        acc += f_norm2(h);
     }
-    const DD s = block_sum<kGB>(DD{acc, 0.0});
-    if (threadIdx.x == 0) {
-        const uint32_t i = blockIdx.y * ntiles + tile;
-        partials[2 * i] = s.hi;
-        partials[2 * i + 1] = s.lo;
+    if constexpr (!STORE) {
+        const DD s = block_sum<kGB>(DD{acc, 0.0});
+        if (threadIdx.x == 0) {
+            const uint32_t i = blockIdx.y * ntiles + tile;
+            partials[2 * i] = s.hi;
+            partials[2 * i + 1] = s.lo;
+        }
     }
 }
 
@@ -238,9 +274,10 @@ size_t rect_fp_bytes(const Lat& L, int is_f32) {
     return (size_t)3 * rect_tiles(L) * 9 * kFC * sz + (size_t)36 * L.V * sz;
 }
 
-// partials: rect_tiles for each of the planes (x, y), (x, z), (z, y)
+// partials: rect_tiles for each of the planes (x, y), (x, z), (z, y).  With Abuf, the per-site
+// AH parts go there instead (the B^2 adjoint's clover sums) and no partials are written.
 hipError_t launch_magnetic_rect(const Lat& L, const void* U, int is_f32, int su3, double* partials, void* fpbuf,
-                                hipStream_t st, uint32_t* npart) {
+                                hipStream_t st, uint32_t* npart, void* Abuf) {
     if (!rect_fits(L) || !fits_buffer(L.V, is_f32)) return hipErrorInvalidValue;
     const uint32_t nt = rect_tiles(L);
     const Lat Lzy = zy_lattice(L);
@@ -264,7 +301,10 @@ hipError_t launch_magnetic_rect(const Lat& L, const void* U, int is_f32, int su3
                 else
                     rect_kernel<F, false, 1><<<g, kRW, 0, st>>>(L, f, Lzy, fzy, nt, fp);
             }
-            rect_gather_kernel<R><<<g, kGB, 0, st>>>(L, Lzy, fp, nt, partials);
+            if (Abuf)
+                rect_gather_kernel<R, true><<<g, kGB, 0, st>>>(L, Lzy, fp, nt, nullptr, (R*)Abuf);
+            else
+                rect_gather_kernel<R, false><<<g, kGB, 0, st>>>(L, Lzy, fp, nt, partials, nullptr);
         }
     });
     if (npart) *npart = 3 * nt;

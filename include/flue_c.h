@@ -340,7 +340,8 @@ int flue_b2_grad_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t 
                     size_t work_bytes, void* stream);
 
 /* gradient of the magnetic B (flue_magnetic); temporal links get 0.  `work` holds the
- * per-site anti-Hermitian parts of the three spatial-plane leaf sums
+ * per-site anti-Hermitian parts of the three spatial-plane leaf sums, then the reverse form's
+ * slots (and, on extents divisible by 16, the rectangle-first tiles that form those sums)
  * (flue_b2_work_bytes; NULL = library buffer).  Precondition in every precision: SU(3)
  * links, since the gather takes the product from a leaf corner to the walk's end as
  * X^+ F.  The call first runs flue_check_field's scan with the precision's bound (waiting
