@@ -170,6 +170,42 @@ __device__ __forceinline__ void store_link(R* __restrict__ p, uint32_t V, int mu
     }
 }
 
+// Streaming accesses: data read or written once per kernel (accumulators, output links,
+// stored plaquette parts) take the non-temporal cache policy, so they do not push out of L2
+// the links and plaquette parts that neighbouring sites read again.
+#ifndef FLUE_NT
+#define FLUE_NT 1
+#endif
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+#if FLUE_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+#if FLUE_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
+// store_link with streaming stores
+template <typename R>
+__device__ __forceinline__ void store_link_stream(R* __restrict__ p, uint32_t V, int mu, uint32_t s, const Mat<R>& m) {
+    typedef R r2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        r2 v;
+        v.x = m.r[e];
+        v.y = m.i[e];
+        st_stream(reinterpret_cast<r2*>(p + (size_t)(mu * 9 + e) * 2 * V) + s, v);
+    }
+}
+
 // ---------------------------------------------------------------- SU(3) algebra
 //
 // Complex multiply-accumulates are written as explicit FMA chains: an expression such as

@@ -151,12 +151,12 @@ __device__ __forceinline__ void store_plaq_ta(R* __restrict__ T, uint32_t V, int
     acc_ah(h, R(1), A, B);
     const R tr = (h.d[0] + h.d[1] + h.d[2]) * R(1.0 / 3.0);
     R* o = T + (size_t)p * 8 * V + s;
-    o[0] = h.d[0] - tr;
-    o[(size_t)V] = h.d[1] - tr;
+    st_stream(o, h.d[0] - tr);
+    st_stream(o + (size_t)V, h.d[1] - tr);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        o[(size_t)(2 + 2 * k) * V] = R(0.5) * h.or_[k];
-        o[(size_t)(3 + 2 * k) * V] = R(0.5) * h.oi[k];
+        st_stream(o + (size_t)(2 + 2 * k) * V, R(0.5) * h.or_[k]);
+        st_stream(o + (size_t)(3 + 2 * k) * V, R(0.5) * h.oi[k]);
     }
 }
 
@@ -285,13 +285,13 @@ __device__ __forceinline__ void flow_update(const Lat& L, const FV& f, typename 
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         double* p = acc + (size_t)(mu * 8 + k) * L.V + s;
-        const double prev = ca != 0.0 ? ca * *p : 0.0;
+        const double prev = ca != 0.0 ? ca * ld_stream(p) : 0.0;
         a[k] = prev - cb * eps * z[k];
-        *p = a[k];
+        st_stream(p, a[k]);
     }
     Mat<double> W = mul(su3_exp(su3_from8(a)), to_double(load_link(f, mu, s)));
     if (reunit) fix_su3(W);
-    store_link(wout, L.V, mu, s, from_double<R>(W));
+    store_link_stream(wout, L.V, mu, s, from_double<R>(W));
 }
 
 // flow_link with the force from the stored plaquette parts

@@ -250,7 +250,14 @@ __global__ void __launch_bounds__(256) zy_relayout_kernel(Lat L, const R* __rest
         for (int e = 0; e < 9; ++e) {
             tile[i][j] = src[(size_t)(mu * 9 + e) * L.V + s];
             __syncthreads();
-            dst[(size_t)(k * 9 + e) * L.V + s2] = tile[j][i];
+            {
+                typedef R r2 __attribute__((ext_vector_type(2)));
+                const P2 v = tile[j][i];
+                r2 w;
+                w.x = v.x;
+                w.y = v.y;
+                st_stream(reinterpret_cast<r2*>(dst) + (size_t)(k * 9 + e) * L.V + s2, w);
+            }
             __syncthreads();
         }
     }
