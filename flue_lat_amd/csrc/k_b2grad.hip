@@ -346,7 +346,7 @@ __device__ __forceinline__ void emit_slot(R* __restrict__ slots, uint32_t nsl, u
     v[7] = R(0.5) * (M.i[5] + M.i[7]);
     R* p = slots + (size_t)slot * 8 * nsl + o;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) p[(size_t)c * nsl] = w * v[c];
+    for (int c = 0; c < 8; ++c) st_stream(p + (size_t)c * nsl, w * v[c]);
 }
 
 // X^+ Z X with X given by two or three rows
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(kBS) b2_slot_sum_kernel(Lat L, const R* __rest
     if (d == 0) {
 #pragma unroll
         for (int e = 0; e < 9; ++e) G.r[e] = G.i[e] = R(0);
-        store_link(grad, L.V, 0, x, G);
+        store_link_stream(grad, L.V, 0, x, G);
         return;
     }
     int c[4];
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(kBS) b2_slot_sum_kernel(Lat L, const R* __rest
     auto add = [&](int pl, int slot, uint32_t base) {
         const R* p = slots + ((size_t)(pl * kRectSlots + slot) * 8) * nsl + (base - s0);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc[k] += (double)p[(size_t)k * nsl];
+        for (int k = 0; k < 8; ++k) acc[k] += (double)ld_stream(p + (size_t)k * nsl);
     };
     // plane where d is the first direction (d = mu): bottom and top sides
     {
@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(kBS) b2_slot_sum_kernel(Lat L, const R* __rest
         G.r[e] = (R)Gd.r[e];
         G.i[e] = (R)Gd.i[e];
     }
-    store_link(grad, L.V, d, x, G);
+    store_link_stream(grad, L.V, d, x, G);
 }
 
 }  // namespace
