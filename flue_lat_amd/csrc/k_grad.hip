@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(kBS) wilson_grad_kernel(Lat L, FV f, BlockOrde
             G.r[e] *= sc;
             G.i[e] *= sc;
         }
-        store_link(grad, L.V, mu, s, G);
+        store_link_stream(grad, L.V, mu, s, G);
     }
 }
 
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(kBS) wilson_grad_pf_kernel(Lat L, FV f, BlockO
         G.r[e] *= sc;
         G.i[e] *= sc;
     }
-    store_link(grad, L.V, mu, s, G);
+    store_link_stream(grad, L.V, mu, s, G);
 }
 
 template <typename FV>
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kBS) pf_walk_kernel(Lat L, FV f, const typenam
                 Z.r[e] *= sc;
                 Z.i[e] *= sc;
             }
-            store_link(out, L.V, mu, s, Z);
+            store_link_stream(out, L.V, mu, s, Z);
         } else {
             flow_update(L, f, out, acc, s, mu, eps, ca, cb, reunit, Z);
         }
