@@ -685,8 +685,9 @@ int flue_magnetic_ex(const int32_t dims[4], const void* U, int32_t prec, int32_t
     if (!U || !d_B) return FLUE_ENULL;
     if (check_prec(prec) || variant < 0 || variant > kMagneticVariants) return FLUE_EINVAL;
     const Lat L = make_lat(dims);
+    // untuned: the rectangle-first form where it applies, else the precision's quadrant form
     if (variant == 0) variant = tuned_variant(FLUE_OP_MAGNETIC, dims, prec);
-    if (variant == 0) variant = magnetic_default_variant(prec == FLUE_F32);
+    if (variant == 0) variant = magnetic_rect_ok(L, prec) ? kMagneticRect : magnetic_default_variant(prec == FLUE_F32);
     // the rectangle-first form needs tileable extents and room for its footprints; otherwise
     // the precision's default form
     if (variant == kMagneticRect &&

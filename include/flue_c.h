@@ -270,7 +270,8 @@ int flue_scalar_gluon_prop(const int32_t dims[4], const void* U, int32_t prec, d
 int flue_magnetic(const int32_t dims[4], const void* U, int32_t prec, double* d_B, void* ws, size_t ws_bytes,
                   void* stream);
 
-/* flue_magnetic with an explicit kernel variant (0 = the tuned or default choice):
+/* flue_magnetic with an explicit kernel variant (0 = the tuned choice, else 7 where it
+ * applies, else the precision's default, 5 for FP32 and 4 for FP64):
  *   1..3  site-major (one thread does the three planes of its site), 1..3 waves per SIMD;
  *   4..6  plane-major (planes run one after another, sites in per-plane L2-friendly
  *         orders), 1..3 waves per SIMD;

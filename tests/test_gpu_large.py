@@ -51,8 +51,11 @@ def check_tiled(FL, prec, k):
     print(f"[large] {prec} tiled x{k}: {dir_bytes(prec, k) / 2 ** 30:.2f} GiB per direction", flush=True)
     g = FL.GaugeField.random(SMALL, 77, 0.3, "fp64").astype(prec)
     big = tiled(FL, g, k)
+    # past the buffer-resource range there is one magnetic form (site-major, variant 2 in FP32
+    # and 1 in FP64): the small field runs the same per-site arithmetic for the comparison
+    v = (2 if prec == "fp32" else 1) if dir_bytes(prec, k) >= 2 ** 32 else 0
     try:
-        B, Bb = g.magnetic(), big.magnetic()
+        B, Bb = g.magnetic(variant=v), big.magnetic(variant=v)
         assert abs(Bb - B) < 1e-12 * B, (B, Bb)
         s, n = g.plaquette(1, 4, 4)
         sb, nb = big.plaquette(1, 4, 4)
