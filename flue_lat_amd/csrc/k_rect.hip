@@ -2,7 +2,7 @@
 //
 // Planes (x, y) and (x, z): one wavefront per 16 x 16 tile of base sites in one plane slice,
 // x along the lanes (16 consecutive sites, so link loads coalesce) and four rows of the second
-// direction per round.  Every base site forms its seven rectangles once (rect_base) and adds
+// direction per loop iteration.  Every base site forms its seven rectangles once (rect_base) and adds
 // the AH parts of the leaves at the four corners of each into the tile's footprint, 19 x 19
 // cells in LDS (the tile plus the three rows and columns its rectangles reach beyond it).
 // The footprint has one wavefront, whose LDS additions execute in program order, so each
@@ -11,7 +11,8 @@
 // direction and diagonally (in that fixed order), and sums |F|^2.  Plane (z, y) has no x
 // direction to coalesce along: its z and y links are first copied with z fastest (an LDS-tiled
 // transpose), which makes it plane (x', y') of the lattice with extents (NT, NZ, NY, NX), and
-// the same two kernels run there.
+// the same two kernels run there.  The gather can also store each site's AH parts instead of
+// summing |F|^2: the clover sums of the B^2 adjoint.
 #include "flue_rect.hpp"
 #include "flue_launch.hpp"
 
