@@ -68,82 +68,6 @@ __device__ __forceinline__ void ah_add(AH<R>& h, const AH<R>& g) {
     }
 }
 
-// AH parts of V^+ A V for the anti-Hermitian A with parts g: A_aa = i d_a, A_ab = o_ab / 2 and
-// A_ba = -conj(o_ab) / 2 for a < b.  V^+ A V is anti-Hermitian too, so its parts are
-// d'_a = Im (V^+ A V)_aa and o'_ab = 2 (V^+ A V)_ab.  V is a full (three-row) matrix.
-template <typename R>
-__device__ __forceinline__ AH<R> ah_conj(const AH<R>& g, const Mat<R>& V) {
-    // off-diagonal entries of A, row-major, (re, im)
-    R ar[9], ai[9];
-    constexpr int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int a = pa[q], b = pb[q];
-        ar[a * 3 + b] = R(0.5) * g.or_[q];
-        ai[a * 3 + b] = R(0.5) * g.oi[q];
-        ar[b * 3 + a] = R(-0.5) * g.or_[q];
-        ai[b * 3 + a] = R(0.5) * g.oi[q];
-    }
-    // T = A V
-    Mat<R> T;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            R tr = -g.d[a] * V.i[a * 3 + j];
-            R ti = g.d[a] * V.r[a * 3 + j];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (k == a) continue;
-                tr = fmad(ar[a * 3 + k], V.r[k * 3 + j], tr);
-                tr = fmad(-ai[a * 3 + k], V.i[k * 3 + j], tr);
-                ti = fmad(ar[a * 3 + k], V.i[k * 3 + j], ti);
-                ti = fmad(ai[a * 3 + k], V.r[k * 3 + j], ti);
-            }
-            T.r[a * 3 + j] = tr;
-            T.i[a * 3 + j] = ti;
-        }
-    }
-    // (V^+ T)_ab = sum_k conj(V_ka) T_kb
-    AH<R> h;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        R im = V.r[a] * T.i[a];
-        im = fmad(-V.i[a], T.r[a], im);
-#pragma unroll
-        for (int k = 1; k < 3; ++k) {
-            im = fmad(V.r[k * 3 + a], T.i[k * 3 + a], im);
-            im = fmad(-V.i[k * 3 + a], T.r[k * 3 + a], im);
-        }
-        h.d[a] = im;
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int a = pa[q], b = pb[q];
-        R xr = V.r[a] * T.r[b];
-        R xi = V.r[a] * T.i[b];
-        xr = fmad(V.i[a], T.i[b], xr);
-        xi = fmad(-V.i[a], T.r[b], xi);
-#pragma unroll
-        for (int k = 1; k < 3; ++k) {
-            xr = fmad(V.r[k * 3 + a], T.r[k * 3 + b], xr);
-            xr = fmad(V.i[k * 3 + a], T.i[k * 3 + b], xr);
-            xi = fmad(V.r[k * 3 + a], T.i[k * 3 + b], xi);
-            xi = fmad(-V.i[k * 3 + a], T.r[k * 3 + b], xi);
-        }
-        h.or_[q] = R(2) * xr;
-        h.oi[q] = R(2) * xi;
-    }
-    return h;
-}
-
-template <bool C2, typename R>
-__device__ __forceinline__ Mat<R> full_rows(const Mat<R>& M) {
-    Mat<R> F = M;
-    if constexpr (C2) fill_row2(F);
-    return F;
-}
-
 // One rectangle closed at both ends of its diagonal: the weighted AH part of the leaf at x
 // (W = CA CB^+) goes to h0 and to the two groups it belongs to; the weighted leaf at the
 // opposite corner (CB^+ CA) is returned.

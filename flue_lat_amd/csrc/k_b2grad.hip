@@ -19,11 +19,13 @@
 //
 // reverse (variant 2, the default): every rectangle is walked once, from its lower-left
 // corner x0, and hands each of its n links its term; see b2_rect_kernel.  About 4n + 7
-// products per rectangle.  The terms go to fixed slots of a buffer, one per (plane,
-// rectangle shape, step of the loop) and base site, and a second pass adds the 52 slots of
-// each link in a fixed order, so the result is deterministic without atomics.  Rectangles
-// lie in spatial planes, so time slices are independent: the slot buffer covers a slab of
-// whole slices and the two passes run slab by slab.
+// products per rectangle; the corner insertions and the per-link terms are anti-Hermitian,
+// so they are carried as their nine anti-Hermitian parts and moved as V^+ A V (ah_conj,
+// cheaper than two full products).  The terms go to fixed slots of a buffer, one per
+// (plane, rectangle shape, step of the loop) and base site, and a second pass adds the 52
+// slots of each link in a fixed order, so the result is deterministic without atomics.
+// Rectangles lie in spatial planes, so time slices are independent: the slot buffer covers a
+// slab of whole slices and the two passes run slab by slab.
 #include "flue_clover.hpp"
 #include "flue_flow.hpp"
 #include "flue_launch.hpp"
@@ -296,12 +298,6 @@ template <> struct RectShape<2, 1> { static constexpr int off = 30; static const
 template <> struct RectShape<1, 3> { static constexpr int off = 36; static constexpr double w = 0.5 * kK4; };
 template <> struct RectShape<3, 1> { static constexpr int off = 44; static constexpr double w = 0.5 * kK4; };
 
-// X A X^+
-template <typename R>
-__device__ __forceinline__ Mat<R> conj_by(const Mat<R>& X, const Mat<R>& A) {
-    return mul_nd(mul(X, A), X);
-}
-
 template <typename R>
 __device__ __forceinline__ Mat<R> madd(const Mat<R>& a, const Mat<R>& b) {
     Mat<R> c;
@@ -313,14 +309,6 @@ __device__ __forceinline__ Mat<R> madd(const Mat<R>& a, const Mat<R>& b) {
     return c;
 }
 
-// all three rows of a chain that may carry two (C2)
-template <bool C2, typename R>
-__device__ __forceinline__ Mat<R> full_rows(const Mat<R>& M) {
-    Mat<R> F = M;
-    if constexpr (C2) fill_row2(F);
-    return F;
-}
-
 // chain step M * U (two rows with C2)
 template <bool C2, typename R>
 __device__ __forceinline__ Mat<R> chain(const Mat<R>& M, const Mat<R>& U) {
@@ -330,30 +318,84 @@ __device__ __forceinline__ Mat<R> chain(const Mat<R>& M, const Mat<R>& U) {
         return mul(M, U);
 }
 
-// slot k of this base: w * (su(3) coordinates of P_TA(X^+ Z X)), coordinates as su3_to8
-template <typename R>
-__device__ __forceinline__ void emit_slot(R* __restrict__ slots, uint32_t nsl, uint32_t o, int slot, R w,
-                                          const Mat<R>& M) {
-    const R tr = (M.i[0] + M.i[4] + M.i[8]) * R(1.0 / 3.0);
-    R v[8];
-    v[0] = M.i[0] - tr;
-    v[1] = M.i[4] - tr;
-    v[2] = R(0.5) * (M.r[1] - M.r[3]);
-    v[3] = R(0.5) * (M.i[1] + M.i[3]);
-    v[4] = R(0.5) * (M.r[2] - M.r[6]);
-    v[5] = R(0.5) * (M.i[2] + M.i[6]);
-    v[6] = R(0.5) * (M.r[5] - M.r[7]);
-    v[7] = R(0.5) * (M.i[5] + M.i[7]);
-    R* p = slots + (size_t)slot * 8 * nsl + o;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) st_stream(p + (size_t)c * nsl, w * v[c]);
+// slot buffer layout: a slot holds 8 coordinates for each of `pitch` bases (the slab's
+// sites rounded up to 64); the 8 coordinates of 64 consecutive bases are 64 adjacent reals
+// each, so one wavefront's coordinates of a slot fill one contiguous 8 x 64 run
+__device__ __forceinline__ uint32_t slot_pitch(uint32_t nsl) { return (nsl + 63u) & ~63u; }
+__device__ __forceinline__ size_t slot_at(int c, uint32_t o) {
+    return ((size_t)(o >> 6) * 8 + (size_t)c) * 64 + (o & 63u);
 }
 
-// X^+ Z X with X given by two or three rows
-template <bool C2, typename R>
-__device__ __forceinline__ Mat<R> sandwich(const Mat<R>& X, const Mat<R>& Z) {
-    const Mat<R> F = full_rows<C2>(X);
-    return mul(mul_dn(F, Z), F);
+// anti-Hermitian parts of M: d_a = Im M_aa, o_ab = M_ab - conj(M_ba)
+template <typename R>
+__device__ __forceinline__ AH<R> ah_of(const Mat<R>& M) {
+    AH<R> h;
+    constexpr int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        h.d[k] = M.i[k * 4];
+        h.or_[k] = M.r[pa[k] * 3 + pb[k]] - M.r[pb[k] * 3 + pa[k]];
+        h.oi[k] = M.i[pa[k] * 3 + pb[k]] + M.i[pb[k] * 3 + pa[k]];
+    }
+    return h;
+}
+
+// the anti-Hermitian matrix with parts h: A_aa = i d_a, A_ab = o_ab / 2, A_ba = -conj(o_ab) / 2
+template <typename R>
+__device__ __forceinline__ Mat<R> ah_mat(const AH<R>& h) {
+    Mat<R> A;
+    constexpr int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        A.r[k * 4] = R(0);
+        A.i[k * 4] = h.d[k];
+        const R orr = h.or_[k] * R(0.5), oi = h.oi[k] * R(0.5);
+        A.r[pa[k] * 3 + pb[k]] = orr;
+        A.i[pa[k] * 3 + pb[k]] = oi;
+        A.r[pb[k] * 3 + pa[k]] = -orr;
+        A.i[pb[k] * 3 + pa[k]] = oi;
+    }
+    return A;
+}
+
+// the stored parts of A_pl(s), as load_ah reads them
+template <typename R>
+__device__ __forceinline__ AH<R> load_ah_parts(const R* __restrict__ Abuf, uint32_t V, int pl, uint32_t s) {
+    const R* p = Abuf + (size_t)pl * 9 * V + s;
+    AH<R> h;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        h.d[k] = p[(size_t)k * V];
+        h.or_[k] = p[(size_t)(3 + k) * V];
+        h.oi[k] = p[(size_t)(6 + k) * V];
+    }
+    return h;
+}
+
+// slot k of this base: w * (su(3) coordinates of the traceless part of the anti-Hermitian
+// matrix with parts h, coordinates as su3_to8)
+template <typename R>
+__device__ __forceinline__ void emit_slot(R* __restrict__ slots, uint32_t nsl, uint32_t o, int slot, R w,
+                                          const AH<R>& h) {
+    const R tr = (h.d[0] + h.d[1] + h.d[2]) * R(1.0 / 3.0);
+    R v[8];
+    v[0] = h.d[0] - tr;
+    v[1] = h.d[1] - tr;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[2 + 2 * k] = R(0.5) * h.or_[k];
+        v[3 + 2 * k] = R(0.5) * h.oi[k];
+    }
+    const uint32_t pitch = slot_pitch(nsl);
+    R* p = slots + (size_t)slot * 8 * pitch;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) st_stream(p + slot_at(c, o), w * v[c]);
+}
+
+// F A F^+ for the stored A at s (anti-Hermitian), F a full matrix
+template <typename R>
+__device__ __forceinline__ Mat<R> corner(const Mat<R>& F, const R* __restrict__ Abuf, uint32_t V, int pl, uint32_t s) {
+    return ah_mat(ah_conj(load_ah_parts(Abuf, V, pl, s), dag(F)));
 }
 
 // One rectangle shape A x B for every base site of the slab [s0, s0 + nsl) and the
@@ -384,7 +426,7 @@ __global__ void __launch_bounds__(kBS) b2_rect_kernel(Lat L, FV f, const typenam
         Tn[k] = (uint32_t)wrapc(cnu + k, L.ext(nu)) * snu;
     }
 #define P_(i, j) (sp + Tm[i] + Tn[j])
-    R* __restrict__ out = slots + (size_t)(pl * kRectSlots + OFF) * 8 * nsl;
+    R* __restrict__ out = slots + (size_t)(pl * kRectSlots + OFF) * 8 * slot_pitch(nsl);
 
     // ---- the loop and its corner sums
     Mat<R> Aa, Aab, Ahat, W0;
@@ -392,27 +434,29 @@ __global__ void __launch_bounds__(kBS) b2_rect_kernel(Lat L, FV f, const typenam
         Mat<R> X = load_link(f, mu, P_(0, 0));  // Line(mu, 1)
 #pragma unroll
         for (int p = 1; p < A; ++p) X = chain<C2>(X, load_link(f, mu, P_(p, 0)));
-        Aa = conj_by(full_rows<C2>(X), load_ah(Abuf, L.V, pl, P_(A, 0)));
+        Aa = corner(full_rows<C2>(X), Abuf, L.V, pl, P_(A, 0));
 #pragma unroll
         for (int q = 0; q < B; ++q) X = chain<C2>(X, load_link(f, nu, P_(A, q)));  // CA(A, B)
         const Mat<R> CA = full_rows<C2>(X);
-        Aab = conj_by(CA, load_ah(Abuf, L.V, pl, P_(A, B)));
+        Aab = corner(CA, Abuf, L.V, pl, P_(A, B));
         Mat<R> Y = load_link(f, nu, P_(0, 0));  // Line(nu, 1)
 #pragma unroll
         for (int q = 1; q < B; ++q) Y = chain<C2>(Y, load_link(f, nu, P_(0, q)));
-        Ahat = conj_by(full_rows<C2>(Y), load_ah(Abuf, L.V, pl, P_(0, B)));
+        Ahat = corner(full_rows<C2>(Y), Abuf, L.V, pl, P_(0, B));
 #pragma unroll
         for (int p = 0; p < A; ++p) Y = chain<C2>(Y, load_link(f, mu, P_(p, B)));  // CB(B, A)
         W0 = mul_nd(CA, full_rows<C2>(Y));
     }
     const Mat<R> A0 = load_ah(Abuf, L.V, pl, P_(0, 0));
-    const Mat<R> A2ab = conj_by(W0, Ahat);
+    const Mat<R> A2ab = ah_mat(ah_conj(ah_of(Ahat), dag(W0)));
     const Mat<R> T3 = madd(madd(A0, Aa), Aab);
-    const Mat<R> Zt = mul(madd(Ahat, T3), W0);                // top, W0-conjugated
-    const Mat<R> Zl = mul(madd(T3, A2ab), W0);                // left, W0-conjugated
+    // the side sums Z enter the slots as X^+ Z X projected on its traceless anti-Hermitian
+    // part; for unitary X that is X^+ AH(Z) X, so only the AH parts of Z are carried
+    const AH<R> Zt = ah_of(mul(madd(Ahat, T3), W0));                  // top, W0-conjugated
+    const AH<R> Zl = ah_of(mul(madd(T3, A2ab), W0));                  // left, W0-conjugated
     const Mat<R> Sab = madd(Aab, A2ab);
-    const Mat<R> Zr = madd(mul(Sab, W0), mul(W0, madd(A0, Aa)));   // right
-    const Mat<R> Zb = madd(mul(madd(Aa, Sab), W0), mul(W0, A0));   // bottom
+    const AH<R> Zr = ah_of(madd(mul(Sab, W0), mul(W0, madd(A0, Aa))));   // right
+    const AH<R> Zb = ah_of(madd(mul(madd(Aa, Sab), W0), mul(W0, A0)));   // bottom
 
     // ---- bottom (steps 0 .. A-1, X = Line(mu, p)) and right (steps A .. A+B-1, X = CA(A, q))
     emit_slot(out, nsl, o, 0, w, Zb);
@@ -420,12 +464,12 @@ __global__ void __launch_bounds__(kBS) b2_rect_kernel(Lat L, FV f, const typenam
         Mat<R> X = load_link(f, mu, P_(0, 0));
 #pragma unroll
         for (int p = 1; p < A; ++p) {
-            emit_slot(out, nsl, o, p, w, sandwich<C2>(X, Zb));
+            emit_slot(out, nsl, o, p, w, ah_conj(Zb, full_rows<C2>(X)));
             X = chain<C2>(X, load_link(f, mu, P_(p, 0)));
         }
 #pragma unroll
         for (int q = 0; q < B; ++q) {
-            emit_slot(out, nsl, o, A + q, w, sandwich<C2>(X, Zr));
+            emit_slot(out, nsl, o, A + q, w, ah_conj(Zr, full_rows<C2>(X)));
             if (q + 1 < B) X = chain<C2>(X, load_link(f, nu, P_(A, q)));
         }
     }
@@ -436,12 +480,12 @@ __global__ void __launch_bounds__(kBS) b2_rect_kernel(Lat L, FV f, const typenam
         Mat<R> Y = load_link(f, nu, P_(0, 0));
 #pragma unroll
         for (int m = 1; m < B; ++m) {
-            emit_slot(out, nsl, o, 2 * A + 2 * B - 1 - m, -w, sandwich<C2>(Y, Zl));
+            emit_slot(out, nsl, o, 2 * A + 2 * B - 1 - m, -w, ah_conj(Zl, full_rows<C2>(Y)));
             Y = chain<C2>(Y, load_link(f, nu, P_(0, m)));
         }
 #pragma unroll
         for (int m = 0; m < A; ++m) {
-            emit_slot(out, nsl, o, 2 * A + B - 1 - m, -w, sandwich<C2>(Y, Zt));
+            emit_slot(out, nsl, o, 2 * A + B - 1 - m, -w, ah_conj(Zt, full_rows<C2>(Y)));
             if (m + 1 < A) Y = chain<C2>(Y, load_link(f, mu, P_(m, B)));
         }
     }
@@ -472,9 +516,10 @@ __global__ void __launch_bounds__(kBS) b2_slot_sum_kernel(Lat L, const R* __rest
     for (int k = 0; k < 8; ++k) acc[k] = 0.0;
     // slab-relative base site of the rectangle whose step `slot` is this link
     auto add = [&](int pl, int slot, uint32_t base) {
-        const R* p = slots + ((size_t)(pl * kRectSlots + slot) * 8) * nsl + (base - s0);
+        const uint32_t pitch = slot_pitch(nsl);
+        const R* p = slots + ((size_t)(pl * kRectSlots + slot) * 8) * pitch;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc[k] += (double)ld_stream(p + (size_t)k * nsl);
+        for (int k = 0; k < 8; ++k) acc[k] += (double)ld_stream(p + slot_at(k, base - s0));
     };
     // plane where d is the first direction (d = mu): bottom and top sides
     {
@@ -554,7 +599,8 @@ uint32_t b2_slab_slices(const Lat& L, int is_f32) {
 }
 
 size_t b2_slot_bytes(const Lat& L, int is_f32) {
-    return (size_t)b2_slab_slices(L, is_f32) * L.Vs * 3 * kRectSlots * 8 * (is_f32 ? 4 : 8);
+    const uint64_t nsl = (uint64_t)b2_slab_slices(L, is_f32) * L.Vs;
+    return (size_t)((nsl + 63) & ~(uint64_t)63) * 3 * kRectSlots * 8 * (is_f32 ? 4 : 8);
 }
 
 hipError_t launch_b2_grad_rev(const Lat& L, const void* U, int is_f32, const void* Abuf, void* slots, void* grad,
