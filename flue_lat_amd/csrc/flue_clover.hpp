@@ -74,11 +74,6 @@ __device__ __forceinline__ void acc_ah(AH<R>& h, R w, const Mat<R>& CA, const Ma
     }
 }
 
-// One quadrant of the clover: the counter-clockwise loops leave x along d1 = S1 * e(ax1)
-// and come back along -d2 = -S2 * e(ax2).  P(a1, a2) = x + a1 d1 + a2 d2 is a lattice
-// point with site index sp + T1[a1] + T2[a2].  A link along d1 that leaves P is
-// U_ax1(P) for S1 > 0 and U_ax1(P + d1)^+ for S1 < 0; the dagger is folded into the
-// product (A U^+) instead of being materialised.
 // all three rows of a chain that may carry two (C2: SU(3) links, row 2 = conj(row0 x row1))
 template <bool C2, typename R>
 __device__ __forceinline__ Mat<R> full_rows(const Mat<R>& M) {
@@ -87,11 +82,10 @@ __device__ __forceinline__ Mat<R> full_rows(const Mat<R>& M) {
     return F;
 }
 
-// AH parts of V^+ A V for the anti-Hermitian A with parts g: A_aa = i d_a, A_ab = o_ab / 2 and
-// A_ba = -conj(o_ab) / 2 for a < b.  V^+ A V is anti-Hermitian too, so its parts are
-// d'_a = Im (V^+ A V)_aa and o'_ab = 2 (V^+ A V)_ab.  V is a full (three-row) matrix.
+// A V for the anti-Hermitian A with parts g: A_aa = i d_a, A_ab = o_ab / 2 and
+// A_ba = -conj(o_ab) / 2 for a < b (the real diagonal of A, zero, takes no multiplies)
 template <typename R>
-__device__ __forceinline__ AH<R> ah_conj(const AH<R>& g, const Mat<R>& V) {
+__device__ __forceinline__ Mat<R> ah_mul(const AH<R>& g, const Mat<R>& V) {
     // off-diagonal entries of A, row-major, (re, im)
     R ar[9], ai[9];
     constexpr int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
@@ -123,6 +117,16 @@ __device__ __forceinline__ AH<R> ah_conj(const AH<R>& g, const Mat<R>& V) {
             T.i[a * 3 + j] = ti;
         }
     }
+    return T;
+}
+
+// AH parts of V^+ A V for the anti-Hermitian A with parts g (as ah_mul).  V^+ A V is
+// anti-Hermitian too, so its parts are d'_a = Im (V^+ A V)_aa and o'_ab = 2 (V^+ A V)_ab.
+// V is a full (three-row) matrix.
+template <typename R>
+__device__ __forceinline__ AH<R> ah_conj(const AH<R>& g, const Mat<R>& V) {
+    constexpr int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    const Mat<R> T = ah_mul(g, V);
     // (V^+ T)_ab = sum_k conj(V_ka) T_kb
     AH<R> h;
 #pragma unroll
@@ -156,6 +160,11 @@ __device__ __forceinline__ AH<R> ah_conj(const AH<R>& g, const Mat<R>& V) {
     return h;
 }
 
+// One quadrant of the clover: the counter-clockwise loops leave x along d1 = S1 * e(ax1)
+// and come back along -d2 = -S2 * e(ax2).  P(a1, a2) = x + a1 d1 + a2 d2 is a lattice
+// point with site index sp + T1[a1] + T2[a2].  A link along d1 that leaves P is
+// U_ax1(P) for S1 > 0 and U_ax1(P + d1)^+ for S1 < 0; the dagger is folded into the
+// product (A U^+) instead of being materialised.
 template <typename FV>
 struct Quad {
     FV f;

@@ -340,24 +340,6 @@ __device__ __forceinline__ AH<R> ah_of(const Mat<R>& M) {
     return h;
 }
 
-// the anti-Hermitian matrix with parts h: A_aa = i d_a, A_ab = o_ab / 2, A_ba = -conj(o_ab) / 2
-template <typename R>
-__device__ __forceinline__ Mat<R> ah_mat(const AH<R>& h) {
-    Mat<R> A;
-    constexpr int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        A.r[k * 4] = R(0);
-        A.i[k * 4] = h.d[k];
-        const R orr = h.or_[k] * R(0.5), oi = h.oi[k] * R(0.5);
-        A.r[pa[k] * 3 + pb[k]] = orr;
-        A.i[pa[k] * 3 + pb[k]] = oi;
-        A.r[pb[k] * 3 + pa[k]] = -orr;
-        A.i[pb[k] * 3 + pa[k]] = oi;
-    }
-    return A;
-}
-
 // the stored parts of A_pl(s), as load_ah reads them
 template <typename R>
 __device__ __forceinline__ AH<R> load_ah_parts(const R* __restrict__ Abuf, uint32_t V, int pl, uint32_t s) {
@@ -392,10 +374,38 @@ __device__ __forceinline__ void emit_slot(R* __restrict__ slots, uint32_t nsl, u
     for (int c = 0; c < 8; ++c) st_stream(p + slot_at(c, o), w * v[c]);
 }
 
-// F A F^+ for the stored A at s (anti-Hermitian), F a full matrix
+// parts of F A F^+ for the stored A at s (anti-Hermitian), F a full matrix
 template <typename R>
-__device__ __forceinline__ Mat<R> corner(const Mat<R>& F, const R* __restrict__ Abuf, uint32_t V, int pl, uint32_t s) {
-    return ah_mat(ah_conj(load_ah_parts(Abuf, V, pl, s), dag(F)));
+__device__ __forceinline__ AH<R> corner(const Mat<R>& F, const R* __restrict__ Abuf, uint32_t V, int pl, uint32_t s) {
+    return ah_conj(load_ah_parts(Abuf, V, pl, s), dag(F));
+}
+
+template <typename R>
+__device__ __forceinline__ AH<R> ah_plus(const AH<R>& a, const AH<R>& b) {
+    AH<R> c;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c.d[k] = a.d[k] + b.d[k];
+        c.or_[k] = a.or_[k] + b.or_[k];
+        c.oi[k] = a.oi[k] + b.oi[k];
+    }
+    return c;
+}
+
+// W A for the anti-Hermitian A with parts h, as -(A W^+)^+
+template <typename R>
+__device__ __forceinline__ Mat<R> mul_ah(const Mat<R>& W, const AH<R>& h) {
+    const Mat<R> T = ah_mul(h, dag(W));
+    Mat<R> M;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            M.r[a * 3 + b] = -T.r[b * 3 + a];
+            M.i[a * 3 + b] = T.i[b * 3 + a];
+        }
+    }
+    return M;
 }
 
 // One rectangle shape A x B for every base site of the slab [s0, s0 + nsl) and the
@@ -429,7 +439,8 @@ __global__ void __launch_bounds__(kBS) b2_rect_kernel(Lat L, FV f, const typenam
     R* __restrict__ out = slots + (size_t)(pl * kRectSlots + OFF) * 8 * slot_pitch(nsl);
 
     // ---- the loop and its corner sums
-    Mat<R> Aa, Aab, Ahat, W0;
+    AH<R> Aa, Aab, Ahat;  // corner sums moved to x0 (anti-Hermitian parts)
+    Mat<R> W0;
     {
         Mat<R> X = load_link(f, mu, P_(0, 0));  // Line(mu, 1)
 #pragma unroll
@@ -447,16 +458,16 @@ __global__ void __launch_bounds__(kBS) b2_rect_kernel(Lat L, FV f, const typenam
         for (int p = 0; p < A; ++p) Y = chain<C2>(Y, load_link(f, mu, P_(p, B)));  // CB(B, A)
         W0 = mul_nd(CA, full_rows<C2>(Y));
     }
-    const Mat<R> A0 = load_ah(Abuf, L.V, pl, P_(0, 0));
-    const Mat<R> A2ab = ah_mat(ah_conj(ah_of(Ahat), dag(W0)));
-    const Mat<R> T3 = madd(madd(A0, Aa), Aab);
+    const AH<R> A0 = load_ah_parts(Abuf, L.V, pl, P_(0, 0));
+    const AH<R> A2ab = ah_conj(Ahat, dag(W0));
+    const AH<R> T3 = ah_plus(ah_plus(A0, Aa), Aab);
     // the side sums Z enter the slots as X^+ Z X projected on its traceless anti-Hermitian
     // part; for unitary X that is X^+ AH(Z) X, so only the AH parts of Z are carried
-    const AH<R> Zt = ah_of(mul(madd(Ahat, T3), W0));                  // top, W0-conjugated
-    const AH<R> Zl = ah_of(mul(madd(T3, A2ab), W0));                  // left, W0-conjugated
-    const Mat<R> Sab = madd(Aab, A2ab);
-    const AH<R> Zr = ah_of(madd(mul(Sab, W0), mul(W0, madd(A0, Aa))));   // right
-    const AH<R> Zb = ah_of(madd(mul(madd(Aa, Sab), W0), mul(W0, A0)));   // bottom
+    const AH<R> Zt = ah_of(ah_mul(ah_plus(Ahat, T3), W0));                        // top, W0-conjugated
+    const AH<R> Zl = ah_of(ah_mul(ah_plus(T3, A2ab), W0));                        // left, W0-conjugated
+    const AH<R> Sab = ah_plus(Aab, A2ab);
+    const AH<R> Zr = ah_of(madd(ah_mul(Sab, W0), mul_ah(W0, ah_plus(A0, Aa))));   // right
+    const AH<R> Zb = ah_of(madd(ah_mul(ah_plus(Aa, Sab), W0), mul_ah(W0, A0)));   // bottom
 
     // ---- bottom (steps 0 .. A-1, X = Line(mu, p)) and right (steps A .. A+B-1, X = CA(A, q))
     emit_slot(out, nsl, o, 0, w, Zb);
