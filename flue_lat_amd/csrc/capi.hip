@@ -128,13 +128,13 @@ size_t b2_region_bytes(const Lat& L, int32_t prec) {
 }
 
 // the B^2 adjoint after the SU(3) check: clover sums into the start of `work` (rectangle-first
-// where it applies), then variant 1 (per-link gather) or 2 (reverse mode with the slot buffer
-// after the clover sums)
+// where it applies, with two-row chains in both precisions since the links are SU(3)), then
+// variant 1 (per-link gather) or 2 (reverse mode with the slot buffer after the clover sums)
 int b2_grad_run(const Lat& L, const void* U, int32_t prec, int variant, void* work, void* grad, hipStream_t st) {
     const int f32 = prec == FLUE_F32;
     void* region = (char*)work + b2_ah_bytes(L, prec);
     if (magnetic_rect_ok(L, prec))
-        FLUE_TRY(launch_magnetic_rect(L, U, f32, 0, nullptr, region, st, nullptr, work));
+        FLUE_TRY(launch_magnetic_rect(L, U, f32, 1, nullptr, region, st, nullptr, work));
     else
         FLUE_TRY(launch_magnetic_ah(L, U, f32, work, st));
     if (variant == 1)
