@@ -273,7 +273,9 @@ __device__ __forceinline__ Mat<typename FV::real> ta_force_walk(const Lat& L, co
     return proj_ta(Z);
 }
 
-// the RK3 update of link (s, mu) given its force Zr = P_TA(U Sigma)
+// the RK3 update of link (s, mu) given its force Zr = P_TA(U Sigma).  reunit marks the step's
+// last stage: the new link is reunitarised, and acc is left unwritten (the next step's first
+// stage does not read it)
 template <typename FV>
 __device__ __forceinline__ void flow_update(const Lat& L, const FV& f, typename FV::real* __restrict__ wout,
                                             double* __restrict__ acc, uint32_t s, int mu, double eps, double ca,
@@ -287,7 +289,7 @@ __device__ __forceinline__ void flow_update(const Lat& L, const FV& f, typename 
         double* p = acc + (size_t)(mu * 8 + k) * L.V + s;
         const double prev = ca != 0.0 ? ca * ld_stream(p) : 0.0;
         a[k] = prev - cb * eps * z[k];
-        st_stream(p, a[k]);
+        if (!reunit) st_stream(p, a[k]);
     }
     Mat<double> W = mul(su3_exp(su3_from8(a)), to_double(load_link(f, mu, s)));
     if (reunit) fix_su3(W);

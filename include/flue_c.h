@@ -353,7 +353,8 @@ int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad,
 
 /* one stage (0, 1, 2) of Luescher's RK3 Wilson flow with step eps, reading Win and
  * writing Wout (distinct buffers); acc is the FP64 accumulator (32 * V doubles) the
- * three stages of a step share.  The exponential map, the product and the
+ * three stages of a step share: stage 0 writes it without reading it, stage 1 reads and
+ * writes it, stage 2 only reads it.  The exponential map, the product and the
  * reunitarisation after stage 2 are done in FP64. */
 int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* Wout, double* acc, double eps,
                     int32_t stage, void* stream);
