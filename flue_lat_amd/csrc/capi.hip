@@ -144,9 +144,13 @@ int b2_grad_run(const Lat& L, const void* U, int32_t prec, int variant, void* wo
     return FLUE_OK;
 }
 
-// flow work buffer: the second link buffer, then the FP64 accumulator, then the plaquette parts
+// flow work buffer: the second link buffer, then the FP64 accumulator, then the plaquette
+// parts, then a third link buffer
 size_t flow_acc_offset(const Lat& L, int32_t prec) {
     return ((size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8) + 255) & ~(size_t)255;
+}
+size_t flow_w3_offset(const Lat& L, int32_t prec) {
+    return flow_acc_offset(L, prec) + (size_t)L.V * 32 * 8 + align256(plaq_ta_bytes(L, prec == FLUE_F32));
 }
 
 // one RK3 stage (Luescher: acc <- ca acc + cb Z, (ca, cb) = (0, 1/4), (-17/9, 8/9), (-1, 3/4))
@@ -764,9 +768,9 @@ size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec) {
 size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec) {
     if (check_dims(dims) || check_prec(prec)) return 0;
     const Lat L = make_lat(dims);
-    // a second link buffer, the FP64 RK3 accumulator (8 su(3) coordinates per link) and the
-    // plaquette parts of the plaquette-first stencil
-    return flow_acc_offset(L, prec) + (size_t)L.V * 32 * 8 + plaq_ta_bytes(L, prec == FLUE_F32);
+    // a second link buffer, the FP64 RK3 accumulator (8 su(3) coordinates per link), the
+    // plaquette parts of the plaquette-first stencil and a third link buffer (an odd step)
+    return flow_w3_offset(L, prec) + flow_acc_offset(L, prec);
 }
 
 int flue_wilson_action(const int32_t dims[4], const void* U, int32_t prec, double beta, double* d_S, void* ws,
@@ -910,14 +914,10 @@ int flue_wilson_flow(const int32_t dims[4], void* U, int32_t prec, double eps, i
             if ((rc = two_steps())) return rc;
     }
     if (nsteps % 2) {
-        // last odd step: run it into W2 and copy back
-        for (int k = 0; k < 3; ++k) {
-            void* in = (k % 2 == 0) ? U : W2;
-            void* out = (k % 2 == 0) ? W2 : U;
-            if ((rc = flow_stage_run(L, in, prec, out, acc, eps, k, variant, T, st))) return rc;
-        }
-        FLUE_TRY(hipMemcpyAsync(U, W2, (size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8), hipMemcpyDeviceToDevice,
-                                st));
+        // last odd step: U -> W2 -> W3 -> U, so its last stage writes U (no copy back)
+        void* bufs[4] = {U, W2, (char*)w + flow_w3_offset(L, prec), U};
+        for (int k = 0; k < 3; ++k)
+            if ((rc = flow_stage_run(L, bufs[k], prec, bufs[k + 1], acc, eps, k, variant, T, st))) return rc;
     }
     return FLUE_OK;
 }
@@ -981,9 +981,8 @@ size_t flue_autotune_work_bytes(int32_t op, const int32_t dims[4], int32_t prec)
     case FLUE_OP_MAGNETIC: return flue_magnetic_work_bytes(dims, prec);
     case FLUE_OP_PLAQUETTE: return partial_bytes(plaquette_max_blocks(L));
     case FLUE_OP_B2_GRAD: return flue_b2_work_bytes(dims, prec) + fb;  // work, then a gradient
-    case FLUE_OP_FLOW:  // two link buffers, accumulator, plaquette parts
-        return flow_acc_offset(L, prec) + (size_t)L.V * 32 * sizeof(double) + plaq_ta_bytes(L, prec == FLUE_F32) +
-               flow_acc_offset(L, prec);
+    case FLUE_OP_FLOW:  // the flow's work buffer
+        return flue_flow_work_bytes(dims, prec);
     default: return 0;
     }
 }
@@ -1016,7 +1015,7 @@ int flue_autotune(int32_t op, const int32_t dims[4], const void* U, int32_t prec
                     // accumulator afresh, so every timed step sees the same values (stages 1
                     // and 2 alone would scale it on every call and slow each variant more).
             void* T = (char*)facc + (size_t)L.V * 32 * 8;
-            void* w2 = (char*)T + plaq_ta_bytes(L, is_f32);
+            void* w2 = (char*)w + flow_w3_offset(L, prec);
             int r = flow_stage_run(L, U, prec, w, facc, 0.01, 0, v, T, st);
             if (!r) r = flow_stage_run(L, w, prec, w2, facc, 0.01, 1, v, T, st);
             if (!r) r = flow_stage_run(L, w2, prec, w, facc, 0.01, 2, v, T, st);

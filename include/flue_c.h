@@ -361,7 +361,10 @@ int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* 
 
 /* nsteps RK3 steps in place on U.  use_graph != 0 captures two steps in a hipGraph and
  * replays it (needs a non-default stream); `work` is flue_flow_work_bytes (NULL = library
- * buffer, which must then exist before any capture).  The flow moves SU(3) links along the
+ * buffer, which must then exist before any capture): two more link buffers, the
+ * accumulator and the plaquette parts.  The stages alternate between U and the second
+ * buffer, and an odd last step passes through the third, so every step ends in U without
+ * a copy.  The flow moves SU(3) links along the
  * group (exponential map, reunitarisation after every step), so it takes SU(3) links in
  * every precision: the tuned FLUE_OP_FLOW variant, including the plaquette-first variants
  * 7..13, is used.  flue_flow_stage (no workspace) uses the staple sums in the tuned
