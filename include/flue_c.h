@@ -296,8 +296,10 @@ size_t flue_magnetic_work_bytes(const int32_t dims[4], int32_t prec);
  * FLUE_EWS before any launch.  Every call on a GPU shares that GPU's array: calls that
  * overlap on one GPU (on different streams or host threads) race, so serialise them.  The
  * kernels write every partial the reduction reads, so no clearing is needed between calls.
- * Results are identical to flue_magnetic_ex with the same variant; variant 7, which needs a
- * footprint workspace, runs as the default form here. */
+ * Results are identical to flue_magnetic_ex with the same variant and an 8 MiB workspace:
+ * variant 7 (also as the tuned choice) runs where its footprints fit the array (extents
+ * divisible by 16 and up to about 28 k sites in FP32, 14 k in FP64) and as the default form
+ * elsewhere. */
 int flue_magnetic_noalloc(const int32_t dims[4], const void* U, int32_t prec, int32_t variant, double* d_B,
                           void* stream);
 
