@@ -3,8 +3,9 @@ device scratch instead of a workspace, for callers that cannot allocate.  Called
 the C ABI and checked against flue_magnetic_ex on the same field given a workspace of the
 scratch's size (8 MiB): the kernels and the fixed-order reduction are the same, so the
 results must agree bit for bit.  (The rectangle-first variant 7 needs more room than that
-and runs as the default form in both.)  Two calls in a row, the second on a smaller
-lattice, show that the shared scratch carries nothing from one call to the next.
+on most lattices and runs as the default form in both there; on 16^3 x 2 its footprints fit
+and it runs in both.)  Two calls in a row, the second on a smaller lattice, show that the
+shared scratch carries nothing from one call to the next.
 """
 import ctypes
 
@@ -59,6 +60,26 @@ def test_noalloc_matches_workspace_entry(FL, prec):
             got, want = run_both(FL, fn, g, variant)
             assert got == want, (dims, variant, got, want)
         assert run_both(FL, fn, g, 7)[0] == g.magnetic(variant=4 if prec == "fp64" else 5)
+        tol = 1e-12 if prec == "fp64" else 1e-6
+        assert abs(got - O.magnetic(U)) <= tol * O.magnetic(U)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_noalloc_rectangle_form_where_it_fits(FL, prec):
+    # extents divisible by 16 and footprints within the scratch (about 4.9 MB in FP64 at
+    # 16^3 x 2): variant 7 runs the rectangle-first kernels on the scratch, then on a smaller
+    # lattice
+    from flue_lat_amd._lib import lib
+    fn = noalloc(lib())
+    for dims, seed in (((2, 16, 16, 16), 63), ((1, 16, 16, 16), 64)):
+        print(f"[noalloc] rect {prec} {dims}", flush=True)
+        U = O.random_field(dims, seed, 0.3)
+        g = FL.GaugeField.from_complex(U, prec)
+        assert lib().flue_magnetic_work_bytes(g._cdims, g.prec) <= SCRATCH
+        got, want = run_both(FL, fn, g, 7)
+        assert got == want, (dims, got, want)
+        # the same kernels and fixed-order sums as with the field's own workspace
+        assert got == g.magnetic(variant=7)
         tol = 1e-12 if prec == "fp64" else 1e-6
         assert abs(got - O.magnetic(U)) <= tol * O.magnetic(U)
 
