@@ -144,7 +144,7 @@ int b2_grad_run(const Lat& L, const void* U, int32_t prec, int variant, void* wo
     return FLUE_OK;
 }
 
-// flow work buffer: the second link buffer, then the FP64 accumulator, then the plaquette
+// flow work buffer: the second link buffer, then the accumulator, then the plaquette
 // parts, then a third link buffer
 size_t flow_acc_offset(const Lat& L, int32_t prec) {
     return ((size_t)L.V * kPlanes * (prec == FLUE_F32 ? 4 : 8) + 255) & ~(size_t)255;
@@ -154,7 +154,7 @@ size_t flow_w3_offset(const Lat& L, int32_t prec) {
 }
 
 // one RK3 stage (Luescher: acc <- ca acc + cb Z, (ca, cb) = (0, 1/4), (-17/9, 8/9), (-1, 3/4))
-int flow_stage_run(const Lat& L, const void* Win, int32_t prec, void* Wout, double* acc, double eps, int stage,
+int flow_stage_run(const Lat& L, const void* Win, int32_t prec, void* Wout, void* acc, double eps, int stage,
                    int variant, void* T, hipStream_t st) {
     static const double ca[3] = {0.0, -17.0 / 9.0, -1.0};
     static const double cb[3] = {0.25, 8.0 / 9.0, 0.75};
@@ -768,7 +768,7 @@ size_t flue_b2_work_bytes(const int32_t dims[4], int32_t prec) {
 size_t flue_flow_work_bytes(const int32_t dims[4], int32_t prec) {
     if (check_dims(dims) || check_prec(prec)) return 0;
     const Lat L = make_lat(dims);
-    // a second link buffer, the FP64 RK3 accumulator (8 su(3) coordinates per link), the
+    // a second link buffer, the RK3 accumulator (8 su(3) coordinates per link, room for FP64), the
     // plaquette parts of the plaquette-first stencil and a third link buffer (an odd step)
     return flow_w3_offset(L, prec) + flow_acc_offset(L, prec);
 }
@@ -1128,7 +1128,7 @@ int ens_grid(const EnsPlan& p, int variant, uint32_t* grid) {
     return FLUE_OK;
 }
 
-// the ensemble work buffer: table, second link buffer, FP64 accumulators, partial sums, and
+// the ensemble work buffer: table, second link buffer, accumulators (room for FP64), partial sums, and
 // the plaquette parts of the plaquette-first stencil
 struct EnsWork {
     EnsEntry* tab;
@@ -1164,7 +1164,7 @@ size_t flue_ensemble_field_bytes(int32_t n, const int32_t* dims, int32_t prec) {
 size_t flue_ensemble_work_bytes(int32_t n, const int32_t* dims, int32_t prec) {
     EnsPlan p;
     if (ens_plan(n, dims, p) || check_prec(prec)) return 0;
-    // table, second link buffer, FP64 accumulators, per-block partial sums, plaquette parts
+    // table, second link buffer, accumulators (room for FP64), per-block partial sums, plaquette parts
     const size_t sz = prec == FLUE_F32 ? 4 : 8;
     return align256(sizeof(EnsEntry) * n) + align256((size_t)p.field_reals * sz) + align256((size_t)p.acc_doubles * 8) +
            align256((size_t)p.total_blocks * 16) + align256((size_t)p.field_reals / 72 * 48 * sz);

@@ -99,13 +99,14 @@ __device__ __forceinline__ Mat<double> su3_from8(const double v[8]) {
 // One RK3 stage for the link U_mu(s), c the coordinates of s (Luescher, JHEP 08 (2010) 071,
 // appendix C):
 //   Z = -P_TA(W Sigma),  acc <- ca * acc + cb * eps * Z,  W_out = exp(acc) W_in
-// acc holds 8 su(3) coordinates per link in FP64 (plane (mu*8+k) of V doubles); the
-// exponential, the product and the reunitarisation (reunit != 0) are FP64.  ca == 0
-// starts a fresh accumulator.  Links are independent within a stage: W_in is only read.
+// acc holds 8 su(3) coordinates per link in the field's precision (plane (mu*8+k) of V
+// reals); the exponential, the product and the reunitarisation (reunit != 0, the step's last
+// stage, which leaves acc unwritten since no later stage reads it) are FP64.  ca == 0 starts
+// a fresh accumulator.  Links are independent within a stage: W_in is only read.
 template <typename FV>
 __device__ __forceinline__ void flow_link(const Lat& L, const FV& f, typename FV::real* __restrict__ wout,
-                                          double* __restrict__ acc, uint32_t s, const int c[4], int mu, double eps,
-                                          double ca, double cb, int reunit) {
+                                          typename FV::real* __restrict__ acc, uint32_t s, const int c[4], int mu,
+                                          double eps, double ca, double cb, int reunit) {
     using R = typename FV::real;
     {
         const Mat<R> U = load_link(f, mu, s);
@@ -114,10 +115,10 @@ __device__ __forceinline__ void flow_link(const Lat& L, const FV& f, typename FV
         su3_to8(Z, z);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            double* p = acc + (size_t)(mu * 8 + k) * L.V + s;
-            const double prev = ca != 0.0 ? ca * *p : 0.0;
+            R* p = acc + (size_t)(mu * 8 + k) * L.V + s;
+            const double prev = ca != 0.0 ? ca * (double)*p : 0.0;
             a[k] = prev - cb * eps * z[k];
-            *p = a[k];
+            if (!reunit) *p = (R)a[k];
         }
         Mat<double> W = mul(su3_exp(su3_from8(a)), to_double(U));
         if (reunit) fix_su3(W);
@@ -278,7 +279,7 @@ __device__ __forceinline__ Mat<typename FV::real> ta_force_walk(const Lat& L, co
 // stage does not read it)
 template <typename FV>
 __device__ __forceinline__ void flow_update(const Lat& L, const FV& f, typename FV::real* __restrict__ wout,
-                                            double* __restrict__ acc, uint32_t s, int mu, double eps, double ca,
+                                            typename FV::real* __restrict__ acc, uint32_t s, int mu, double eps, double ca,
                                             double cb, int reunit, const Mat<typename FV::real>& Zr) {
     using R = typename FV::real;
     const Mat<double> Z = to_double(Zr);
@@ -286,10 +287,10 @@ __device__ __forceinline__ void flow_update(const Lat& L, const FV& f, typename 
     su3_to8(Z, z);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        double* p = acc + (size_t)(mu * 8 + k) * L.V + s;
-        const double prev = ca != 0.0 ? ca * ld_stream(p) : 0.0;
+        R* p = acc + (size_t)(mu * 8 + k) * L.V + s;
+        const double prev = ca != 0.0 ? ca * (double)ld_stream(p) : 0.0;
         a[k] = prev - cb * eps * z[k];
-        if (!reunit) st_stream(p, a[k]);
+        if (!reunit) st_stream(p, (R)a[k]);
     }
     Mat<double> W = mul(su3_exp(su3_from8(a)), to_double(load_link(f, mu, s)));
     if (reunit) fix_su3(W);
@@ -299,7 +300,7 @@ __device__ __forceinline__ void flow_update(const Lat& L, const FV& f, typename 
 // flow_link with the force from the stored plaquette parts
 template <typename FV>
 __device__ __forceinline__ void flow_link_pf(const Lat& L, const FV& f, const typename FV::real* __restrict__ T,
-                                             typename FV::real* __restrict__ wout, double* __restrict__ acc,
+                                             typename FV::real* __restrict__ wout, typename FV::real* __restrict__ acc,
                                              uint32_t s, const int c[4], int mu, double eps, double ca, double cb,
                                              int reunit) {
     flow_update(L, f, wout, acc, s, mu, eps, ca, cb, reunit, ta_force(L, f, T, s, c, mu));

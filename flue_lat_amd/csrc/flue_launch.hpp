@@ -89,7 +89,7 @@ hipError_t launch_magnetic_rect(const Lat& L, const void* U, int is_f32, int su3
                                 hipStream_t st, uint32_t* npart, void* Abuf = nullptr);
 
 // gradients and flow (k_grad.hip, k_b2grad.hip); grad / Wout / Abuf are SoA in the field
-// precision, acc is FP64 (8 su(3) coordinates x 4 directions x V)
+// precision, and so is acc (8 su(3) coordinates x 4 directions x V)
 uint32_t grad_blocks(const Lat& L);
 // stencil variants (k_grad.hip): 1 = staple sums in storage order, 2..6 = staple sums in
 // (z, t) brick orders, 7 = plaquette-first (the six plaquette parts of every site into T,
@@ -109,7 +109,7 @@ constexpr int staple_order_of(int variant) {
 size_t plaq_ta_bytes(const Lat& L, int is_f32);
 hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double beta, void* grad, hipStream_t st,
                               int variant, void* T);
-hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, double* acc, double eps,
+hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, void* acc, double eps,
                              double ca, double cb, int reunit, hipStream_t st, int variant, void* T);
 hipError_t launch_magnetic_ah(const Lat& L, const void* U, int is_f32, void* Abuf, hipStream_t st);
 hipError_t launch_b2_grad(const Lat& L, const void* U, int is_f32, const void* Abuf, void* grad, hipStream_t st);
@@ -128,7 +128,8 @@ hipError_t launch_finalize_affine(const double* partials, uint32_t n, double* ou
                                   hipStream_t st);
 
 // ensembles (k_ensemble.hip): configuration c's field starts at element `off` of the
-// back-to-back fields, its FP64 flow accumulator at `acc_off`, and it owns logical
+// back-to-back fields, its flow accumulator (reals of the fields' precision) at element
+// `acc_off`, and it owns logical
 // blocks [blk0, blk0 + nblk) of 256 sites
 struct EnsEntry {
     Lat L;
@@ -140,7 +141,7 @@ struct EnsEntry {
 // T != nullptr: the plaquette-first stencil with the plaquette parts of every field in T
 // (48 reals per site, field c's at 2/3 of its field offset); T == nullptr: staple sums
 hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* win,
-                                 int is_f32, void* wout, double* acc, double eps, double ca, double cb, int reunit,
+                                 int is_f32, void* wout, void* acc, double eps, double ca, double cb, int reunit,
                                  uint32_t grid, void* T, hipStream_t st);
 hipError_t launch_ens_sum(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* fld, int is_f32,
                           int obs, double* partials, uint32_t grid, hipStream_t st);

@@ -55,7 +55,7 @@ __device__ __forceinline__ void my_range(uint32_t total, uint32_t chunk, uint32_
 template <typename R, bool BUF>
 __global__ void __launch_bounds__(kBS) ens_flow_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
                                                        uint32_t chunk, const R* __restrict__ win, R* __restrict__ wout,
-                                                       double* __restrict__ acc, double eps, double ca, double cb,
+                                                       R* __restrict__ acc, double eps, double ca, double cb,
                                                        int reunit) {
     // units = (logical block, direction), direction fastest: one thread per link, and the
     // four units of a block, which read the same neighbour links, run close together
@@ -96,7 +96,7 @@ template <typename R, bool BUF>
 __global__ void __launch_bounds__(kBS) ens_flow_pf_kernel(const EnsEntry* __restrict__ tab, int n, uint32_t total,
                                                           uint32_t chunk, const R* __restrict__ win,
                                                           const R* __restrict__ T, R* __restrict__ wout,
-                                                          double* __restrict__ acc, double eps, double ca, double cb,
+                                                          R* __restrict__ acc, double eps, double ca, double cb,
                                                           int reunit) {
     uint32_t lo, hi;
     my_range(4u * total, chunk, lo, hi);
@@ -176,7 +176,7 @@ void with_real(int is_f32, bool buf, Fn&& fn) {
 }  // namespace
 
 hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uint32_t maxV, const void* win,
-                                 int is_f32, void* wout, double* acc, double eps, double ca, double cb, int reunit,
+                                 int is_f32, void* wout, void* acc, double eps, double ca, double cb, int reunit,
                                  uint32_t grid, void* T, hipStream_t st) {
     with_real(is_f32, fits_buffer(maxV, is_f32), [&](auto r, auto buf) {
         using R = decltype(r);
@@ -185,10 +185,10 @@ hipError_t launch_ens_flow_stage(const EnsEntry* tab, int n, uint32_t total, uin
         if (T) {
             ens_plaq_ta_kernel<R, B><<<grid, kBS, 0, st>>>(tab, n, total, (total + grid - 1) / grid, (const R*)win, (R*)T);
             ens_flow_pf_kernel<R, B><<<grid, kBS, 0, st>>>(tab, n, total, chunk4, (const R*)win, (const R*)T, (R*)wout,
-                                                           acc, eps, ca, cb, reunit);
+                                                           (R*)acc, eps, ca, cb, reunit);
         } else {
-            ens_flow_kernel<R, B><<<grid, kBS, 0, st>>>(tab, n, total, chunk4, (const R*)win, (R*)wout, acc, eps, ca,
-                                                        cb, reunit);
+            ens_flow_kernel<R, B><<<grid, kBS, 0, st>>>(tab, n, total, chunk4, (const R*)win, (R*)wout, (R*)acc, eps,
+                                                        ca, cb, reunit);
         }
     });
     return hipGetLastError();

@@ -64,8 +64,8 @@ __global__ void __launch_bounds__(kBS) wilson_grad_kernel(Lat L, FV f, BlockOrde
 
 template <typename FV>
 __global__ void __launch_bounds__(kBS) flow_stage_kernel(Lat L, FV f, BlockOrder bo, typename FV::real* __restrict__ wout,
-                                                         double* __restrict__ acc, double eps, double ca, double cb,
-                                                         int reunit) {
+                                                         typename FV::real* __restrict__ acc, double eps, double ca,
+                                                         double cb, int reunit) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int mu = (int)(blk & 3u);
     const uint32_t s = site_block(L, bo, blk >> 2) * kBS + threadIdx.x;
@@ -108,8 +108,8 @@ template <typename FV>
 __global__ void __launch_bounds__(kBS) flow_stage_pf_kernel(Lat L, FV f, BlockOrder bo,
                                                             const typename FV::real* __restrict__ T,
                                                             typename FV::real* __restrict__ wout,
-                                                            double* __restrict__ acc, double eps, double ca, double cb,
-                                                            int reunit) {
+                                                            typename FV::real* __restrict__ acc, double eps, double ca,
+                                                            double cb, int reunit) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int mu = (int)(blk & 3u);
     const uint32_t s = site_block(L, bo, blk >> 2) * kBS + threadIdx.x;
@@ -168,8 +168,8 @@ __global__ void __launch_bounds__(kBS) plaq_ta_walk_kernel(Lat L, FV f, uint32_t
 template <bool GRAD, typename FV>
 __global__ void __launch_bounds__(kBS) pf_walk_kernel(Lat L, FV f, const typename FV::real* __restrict__ T,
                                                       uint32_t nbs, typename FV::real* __restrict__ out,
-                                                      double* __restrict__ acc, double eps, double ca, double cb,
-                                                      int reunit, double beta) {
+                                                      typename FV::real* __restrict__ acc, double eps, double ca,
+                                                      double cb, int reunit, double beta) {
     using R = typename FV::real;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int mu = (int)(blk & 3u);
@@ -252,7 +252,7 @@ hipError_t launch_wilson_grad(const Lat& L, const void* U, int is_f32, double be
     return hipGetLastError();
 }
 
-hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, double* acc, double eps,
+hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wout, void* acc_, double eps,
                              double ca, double cb, int reunit, hipStream_t st, int variant, void* T) {
     const uint32_t g = grad_blocks(L);
     if (flow_is_pf(variant) && !T) return hipErrorInvalidValue;
@@ -261,6 +261,7 @@ hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wo
     with_view(L.V, Win, is_f32, [&](auto f) {
         using F = decltype(f);
         using R = typename F::real;
+        R* acc = (R*)acc_;
         if (variant == kFlowPFWalk) {
             plaq_ta_walk_kernel<F><<<nbs * ntc, kBS, 0, st>>>(L, f, nbs, (R*)T);
             pf_walk_kernel<false, F><<<4 * nbs * ntc, kBS, 0, st>>>(L, f, (const R*)T, nbs, (R*)Wout, acc, eps, ca, cb,

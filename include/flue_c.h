@@ -354,10 +354,11 @@ int flue_b2_grad(const int32_t dims[4], const void* U, int32_t prec, void* grad,
                  void* stream);
 
 /* one stage (0, 1, 2) of Luescher's RK3 Wilson flow with step eps, reading Win and
- * writing Wout (distinct buffers); acc is the FP64 accumulator (32 * V doubles) the
- * three stages of a step share: stage 0 writes it without reading it, stage 1 reads and
- * writes it, stage 2 only reads it.  The exponential map, the product and the
- * reunitarisation after stage 2 are done in FP64. */
+ * writing Wout (distinct buffers); acc is the accumulator the three stages of a step
+ * share, 32 * V reals of the field's precision (doubles for FLUE_F64 and FLUE_F64_SU3,
+ * floats for FLUE_F32; a buffer of 32 * V doubles serves every precision): stage 0 writes
+ * it without reading it, stage 1 reads and writes it, stage 2 only reads it.  The
+ * exponential map, the product and the reunitarisation after stage 2 are done in FP64. */
 int flue_flow_stage(const int32_t dims[4], const void* Win, int32_t prec, void* Wout, double* acc, double eps,
                     int32_t stage, void* stream);
 
