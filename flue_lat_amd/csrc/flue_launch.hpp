@@ -114,9 +114,10 @@ hipError_t launch_flow_stage(const Lat& L, const void* Win, int is_f32, void* Wo
 hipError_t launch_magnetic_ah(const Lat& L, const void* U, int is_f32, void* Abuf, hipStream_t st);
 hipError_t launch_b2_grad(const Lat& L, const void* U, int is_f32, const void* Abuf, void* grad, hipStream_t st);
 // reverse-mode B^2 adjoint: slab by slab of time slices, per-rectangle terms into `slots`
-// (b2_slot_bytes), then a fixed-order sum per link; slabs are sized so the slot buffer
-// stays within kB2SlotBudget (at least one slice)
-constexpr uint64_t kB2SlotBudget = (uint64_t)4 << 30;
+// (b2_slot_bytes), then a fixed-order sum per link; slabs are sized so the slot buffer holds
+// at most kB2SlotReals reals (4 GiB in FP64, 2 GiB in FP32; at least one slice).  At 48^3
+// that is 3 slices in both precisions, which ran faster than 1, 7 or 14.
+constexpr uint64_t kB2SlotReals = (uint64_t)1 << 29;
 // B^2 adjoint forms: 1 = per-link gather, 2 = reverse mode (default)
 constexpr int kB2Variants = 2;
 constexpr int kB2DefaultVariant = 2;

@@ -602,8 +602,8 @@ hipError_t launch_magnetic_ah(const Lat& L, const void* U, int is_f32, void* Abu
 }
 
 uint32_t b2_slab_slices(const Lat& L, int is_f32) {
-    const uint64_t per_slice = (uint64_t)L.Vs * 3 * kRectSlots * 8 * (is_f32 ? 4 : 8);
-    uint64_t S = kB2SlotBudget / per_slice;
+    const uint64_t per_slice = (uint64_t)L.Vs * 3 * kRectSlots * 8;  // slot reals per time slice
+    uint64_t S = kB2SlotReals / per_slice;
     if (S < 1) S = 1;
     if (S > (uint64_t)L.n[0]) S = (uint64_t)L.n[0];
     return (uint32_t)S;
